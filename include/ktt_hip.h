@@ -1,0 +1,148 @@
+/* ktt_hip.h - C ABI of the MI355X (gfx950) direct Coulomb-summation library.
+ *
+ * The reference exports C++ classes only (Source/KttPlatform.h:22-24: KTT_API is empty on
+ * Linux) and has no C ABI. Two layers are offered here:
+ *   1. The ktt C++ API itself (ktt_amd/csrc/ktt/Ktt.h, ktt::Tuner with ComputeApi::HIP), the
+ *      drop-in for Source/Tuner.h; the functions in section 4 drive it for callers that can
+ *      only bind C (ctypes, cgo, JNI).
+ *   2. Direct entry points for the hot path (section 3): one launch of a Coulomb kernel on
+ *      device buffers, compiled through hipRTC with the configuration given as defines.
+ *
+ * Conventions. Every function returns 0 on success and a negative value on failure; the
+ * message of the last failure on the calling thread is available from ktt_hip_last_error().
+ * Atoms are float4 records {x, y, z, q} (Examples/CoulombSum3d/CoulombSum3d.cpp:95-98).
+ * Grids are float32, z-major: index z*G*G + y*G + x (Examples/CoulombSum3d/CoulombSum3d.cu:10-11);
+ * grid point (x, y, z) sits at (h*x, h*y, h*z). All kernels ADD into the output grid ("+="),
+ * as the reference kernels do (CoulombSum3d.cu:46, CoulombSum2d.cl:945); clear it first for V.
+ * A configuration string is a comma-separated list NAME=VALUE of kernel defines, or NULL for
+ * the tuned default; unknown names are passed to the compiler unchanged.
+ * A stream argument is a hipStream_t (NULL: the null stream). kernel_ms, when not NULL,
+ * receives the kernel time measured with HIP events on that stream (the call then waits).
+ */
+#ifndef KTT_HIP_H
+#define KTT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. Library and device ---------------------------------------------------------------- */
+
+/* ABI version of this header (major * 100 + minor). */
+int ktt_hip_abi_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* ktt_hip_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU; not an error). */
+int ktt_hip_device_count(int* count);
+
+/* Name, gfx architecture, compute units and global memory of the current device. */
+int ktt_hip_device_info(char* name, size_t name_length, char* architecture, size_t architecture_length,
+                        int* compute_units, size_t* global_memory);
+
+/* Directory the kernel sources are read from (defaults to the kernels/ directory next to the
+ * library; overridden by the environment variable KTT_HIP_KERNEL_DIR). */
+const char* ktt_hip_kernel_directory(void);
+
+/* hipRTC compilation of one kernel of one source file for gfx950, without loading it; needs
+ * no GPU. log (optional) receives the compiler log. Replaces nothing in the reference: it is
+ * the build check of the kernels. */
+int ktt_hip_compile_check(const char* source_file, const char* kernel_name, const char* config,
+                          char* log, size_t log_length);
+
+/* ---- 2. Device memory and timing plumbing (callers without their own HIP binding) --------- */
+
+int ktt_hip_malloc(void** pointer, size_t bytes);
+int ktt_hip_free(void* pointer);
+int ktt_hip_memcpy_htod(void* device_destination, const void* host_source, size_t bytes);
+int ktt_hip_memcpy_dtoh(void* host_destination, const void* device_source, size_t bytes);
+/* Zero-fills bytes of device memory with a kernel (no hipMemset node, safe to capture). */
+int ktt_hip_zero(void* device_pointer, size_t bytes, void* stream);
+int ktt_hip_synchronize(void);
+int ktt_hip_stream_create(void** stream);
+int ktt_hip_stream_destroy(void* stream);
+/* Event pair for timing a region of work on one stream. */
+int ktt_hip_event_create(void** event);
+int ktt_hip_event_destroy(void* event);
+int ktt_hip_event_record(void* event, void* stream);
+int ktt_hip_event_elapsed_ms(void* start, void* end, float* milliseconds);
+
+/* ---- 3. Coulomb hot path on device buffers ----------------------------------------------- */
+
+/* Forms of the 3D computation. */
+#define KTT_COULOMB_PLAIN 0        /* FP32 accumulation, as the reference kernels */
+#define KTT_COULOMB_COMPENSATED 1  /* tile-relative displacements, two-sum per atom tile,
+                                      FP64 merge of atom-block partials in the same launch */
+
+/* CoulombSum3d, one launch over the whole grid: energy_grid[G^3] += V.
+ * Replaces the launch of directCoulombSum, Examples/CoulombSum3d/CoulombSum3d.cu:1-47
+ * (driven by Examples/CoulombSum3d/CoulombSum3d.cpp:117-237).
+ * atoms: float4[n_atoms] on the device. Config names: WORK_GROUP_SIZE_X, WORK_GROUP_SIZE_Y,
+ * POINTS_X, ROWS_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR, ATOM_BLOCKS. */
+int ktt_coulomb3d(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
+                  int form, const char* config, void* stream, float* kernel_ms);
+
+/* The one-point-per-thread validation kernel (Examples/CoulombSum3d/CoulombSum3dReference.cu:1-28). */
+int ktt_coulomb3d_reference(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
+                            void* stream, float* kernel_ms);
+
+/* CoulombSum2d, one slice: energy_grid[grid_y][grid_x] += V, where atoms[i].z holds the
+ * squared z distance of atom i to the slice (Examples/CoulombSum2d/CoulombSum2d.cl:35-1026).
+ * grid_x must be a multiple of WORK_GROUP_SIZE_X*POINTS_X and grid_y of
+ * WORK_GROUP_SIZE_Y*ROWS_PER_THREAD. */
+int ktt_coulomb2d(const float* atoms, int n_atoms, float spacing, int grid_x, int grid_y, float* energy_grid,
+                  const char* config, void* stream, float* kernel_ms);
+
+/* The 2D validation kernel, "=" store (Examples/CoulombSum2d/CoulombSum2dReference.cl:1-21). */
+int ktt_coulomb2d_reference(const float* atoms, int n_atoms, float spacing, int grid_x, int grid_y,
+                            float* energy_grid, void* stream, float* kernel_ms);
+
+/* CoulombSum3dIterative with the reference launcher's semantics
+ * (Examples/CoulombSum3dIterative/CoulombSum3dIterative.cpp:146-184): for every slice z the
+ * host computes dz^2 = (h*z - Z_i)^2 in FP32 for all atoms, uploads the atom records with .z
+ * replaced, and launches the slice kernel (CoulombSum3dIterative.cl:35) with zIndex = z.
+ * host_atoms: float4[n_atoms] in host memory with the real z. wall_ms receives the host
+ * time of the whole loop, kernel_ms the sum of the slice kernels' event times. */
+int ktt_coulomb3d_iterative(const float* host_atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
+                            const char* config, void* stream, float* kernel_ms, float* wall_ms);
+
+/* Fused slice loop: the slice kernel forms dz^2 on the device (atoms with their real z on
+ * the device), one launch per slice, no host work between launches. */
+int ktt_coulomb3d_slices(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
+                         const char* config, void* stream, float* kernel_ms);
+
+/* Launch geometry a configuration implies for a grid (for callers that size scratch or report
+ * work per workgroup): grid of workgroups and workgroup shape. kind: 3 = CoulombSum3d,
+ * 2 = CoulombSum2d / slice kernels (rows = grid_size). */
+int ktt_coulomb_launch_shape(int kind, int grid_size, const char* config, unsigned int grid[3], unsigned int block[3]);
+
+/* ---- 4. ktt::Tuner on ComputeApi::HIP (host buffers, full framework path) ---------------- */
+
+/* Searchers accepted by the tuning entry points. */
+#define KTT_SEARCHER_DETERMINISTIC 0
+#define KTT_SEARCHER_RANDOM 1
+#define KTT_SEARCHER_MCMC 2
+
+/* Tunes the CoulombSum3d kernel with ktt::Tuner(0, 0, ComputeApi::HIP) on host atom data,
+ * mirroring Examples/CoulombSum3d/CoulombSum3d.cpp:117-237 with this library's parameter
+ * space: every configuration is compiled by hipRTC, timed with HIP events and validated
+ * against the reference kernel with SideBySideComparison at the given tolerance.
+ * max_configurations <= 0 explores the whole space. results_file (optional, without
+ * extension) receives the KTT JSON results; best_config (optional) the best configuration
+ * as NAME=VALUE,...; best_ms (optional) its kernel time; valid_count (optional) the number
+ * of configurations with status Ok; tested_count (optional) the number tested.
+ * energy_grid (optional, host, G^3 floats) receives the output of the best configuration. */
+int ktt_coulomb3d_tune(const float* host_atoms, int n_atoms, float spacing, int grid_size, int searcher,
+                       int max_configurations, double tolerance, const char* results_file,
+                       char* best_config, size_t best_config_length, double* best_ms,
+                       int* valid_count, int* tested_count, float* energy_grid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KTT_HIP_H */

@@ -1,0 +1,123 @@
+"""Builds the native parts of ktt_amd in-tree.
+
+* ``ktt_amd/libktt_hip.so``: the ktt C++ API with the HIP compute engine and the C ABI of
+  ``include/ktt_hip.h`` (g++, linked against the ROCm HIP runtime and hipRTC).
+* ``ktt_amd/bin/``: the example programs (CoulombSum2d, CoulombSum3d, CoulombSum3dIterative)
+  written against the ktt C++ API.
+* every kernel source under ``ktt_amd/kernels`` is compiled once for gfx950 through the
+  library's own hipRTC path (``ktt_hip_compile_check``), which needs no GPU.
+
+Run ``python -m ktt_amd.build`` or call :func:`build`.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures
+import ctypes
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PACKAGE = Path(__file__).resolve().parent
+REPO = PACKAGE.parent
+CSRC = PACKAGE / "csrc"
+KERNELS = PACKAGE / "kernels"
+OBJ_DIR = PACKAGE / "_build"
+LIBRARY = PACKAGE / "libktt_hip.so"
+BIN_DIR = PACKAGE / "bin"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+
+CXX_FLAGS = [
+    "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+    f"-I{CSRC / 'ktt'}", f"-I{CSRC / 'src'}", f"-I{ROCM / 'include'}",
+]
+LINK_FLAGS = [f"-L{ROCM / 'lib'}", "-lamdhip64", "-lhiprtc", f"-Wl,-rpath,{ROCM / 'lib'}", "-ldl", "-pthread"]
+
+# (source file, kernel) pairs compiled as the build check of the device code.
+KERNEL_CHECKS = [
+    ("CoulombSum3d.hip", "directCoulombSum", None),
+    ("CoulombSum3d.hip", "directCoulombSumCompensated", None),
+    ("CoulombSum3dReference.hip", "directCoulombSumReference", None),
+    ("CoulombSum2d.hip", "directCoulombSum", "POINTS_X=2,ROWS_PER_THREAD=8"),
+    ("CoulombSum2dReference.hip", "directCoulombSumReference", None),
+    ("CoulombSum3dIterative.hip", "directCoulombSum", "POINTS_X=2,ROWS_PER_THREAD=8"),
+    ("CoulombSum3dIterative.hip", "directCoulombSumSlice", "POINTS_X=2,ROWS_PER_THREAD=8"),
+    ("CoulombSum3dIterativeReference.hip", "directCoulombSumReference", None),
+    ("coulomb_util.hip", "kttZeroFill16", None),
+]
+
+EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative"]
+
+
+def _run(command: list[str]) -> None:
+    result = subprocess.run(command, capture_output=True, text=True)
+    if result.returncode != 0:
+        raise RuntimeError(f"command failed ({result.returncode}): {' '.join(command)}\n{result.stdout}\n{result.stderr}")
+
+
+def _compile_object(source: Path) -> Path:
+    OBJ_DIR.mkdir(exist_ok=True)
+    target = OBJ_DIR / (source.stem + ".o")
+    _run(["g++", *CXX_FLAGS, "-c", str(source), "-o", str(target)])
+    return target
+
+
+def build_library(jobs: int = 8) -> Path:
+    sources = sorted((CSRC / "src").glob("*.cpp"))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        objects = list(pool.map(_compile_object, sources))
+    temporary = LIBRARY.with_suffix(".so.tmp")
+    _run(["g++", "-shared", "-o", str(temporary), *map(str, objects), *LINK_FLAGS])
+    os.replace(temporary, LIBRARY)
+    return LIBRARY
+
+
+def build_examples() -> list[Path]:
+    BIN_DIR.mkdir(exist_ok=True)
+    built = []
+    for name in EXAMPLES:
+        source = CSRC / "examples" / f"{name}.cpp"
+        if not source.exists():
+            continue
+        target = BIN_DIR / name
+        _run(["g++", *CXX_FLAGS, str(source), "-o", str(target), f"-L{PACKAGE}", "-lktt_hip",
+              f"-Wl,-rpath,{PACKAGE}", *LINK_FLAGS])
+        built.append(target)
+    return built
+
+
+def check_kernels() -> None:
+    library = ctypes.CDLL(str(LIBRARY))
+    library.ktt_hip_compile_check.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+    library.ktt_hip_last_error.restype = ctypes.c_char_p
+    log = ctypes.create_string_buffer(65536)
+    failures = []
+    for source, kernel, config in KERNEL_CHECKS:
+        path = KERNELS / source
+        if not path.exists():
+            failures.append(f"{source}: missing")
+            continue
+        status = library.ktt_hip_compile_check(str(path).encode(), kernel.encode(),
+                                               config.encode() if config else None, log, len(log))
+        if status != 0:
+            failures.append(f"{source}:{kernel}: {library.ktt_hip_last_error().decode()}")
+    if failures:
+        raise RuntimeError("kernel compilation failed:\n" + "\n".join(failures))
+
+
+def build(verbose: bool = True) -> None:
+    if verbose:
+        print("[ktt_amd.build] compiling libktt_hip.so", flush=True)
+    build_library()
+    if verbose:
+        print("[ktt_amd.build] compiling examples", flush=True)
+    build_examples()
+    if verbose:
+        print("[ktt_amd.build] compiling kernels for gfx950 through hipRTC", flush=True)
+    check_kernels()
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,270 @@
+"""ctypes binding of the C ABI in include/ktt_hip.h (libktt_hip.so).
+
+The library is loaded from this package directory only. If it is missing the import fails
+with an error naming the build command: there is no fallback of any kind.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PACKAGE = Path(__file__).resolve().parent
+LIBRARY_PATH = PACKAGE / "libktt_hip.so"
+
+PLAIN = 0
+COMPENSATED = 1
+SEARCHER_DETERMINISTIC = 0
+SEARCHER_RANDOM = 1
+SEARCHER_MCMC = 2
+
+# Every function declared in include/ktt_hip.h: name -> (restype, argtypes).
+_c = ctypes
+_vp = _c.c_void_p
+_fp = _c.POINTER(_c.c_float)
+_uint3 = _c.c_uint * 3
+SIGNATURES = {
+    "ktt_hip_abi_version": (_c.c_int, []),
+    "ktt_hip_last_error": (_c.c_char_p, []),
+    "ktt_hip_device_count": (_c.c_int, [_c.POINTER(_c.c_int)]),
+    "ktt_hip_device_info": (_c.c_int, [_c.c_char_p, _c.c_size_t, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_int),
+                                       _c.POINTER(_c.c_size_t)]),
+    "ktt_hip_kernel_directory": (_c.c_char_p, []),
+    "ktt_hip_compile_check": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_size_t]),
+    "ktt_hip_malloc": (_c.c_int, [_c.POINTER(_vp), _c.c_size_t]),
+    "ktt_hip_free": (_c.c_int, [_vp]),
+    "ktt_hip_memcpy_htod": (_c.c_int, [_vp, _vp, _c.c_size_t]),
+    "ktt_hip_memcpy_dtoh": (_c.c_int, [_vp, _vp, _c.c_size_t]),
+    "ktt_hip_zero": (_c.c_int, [_vp, _c.c_size_t, _vp]),
+    "ktt_hip_synchronize": (_c.c_int, []),
+    "ktt_hip_stream_create": (_c.c_int, [_c.POINTER(_vp)]),
+    "ktt_hip_stream_destroy": (_c.c_int, [_vp]),
+    "ktt_hip_event_create": (_c.c_int, [_c.POINTER(_vp)]),
+    "ktt_hip_event_destroy": (_c.c_int, [_vp]),
+    "ktt_hip_event_record": (_c.c_int, [_vp, _vp]),
+    "ktt_hip_event_elapsed_ms": (_c.c_int, [_vp, _vp, _fp]),
+    "ktt_coulomb3d": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_int, _c.c_char_p, _vp, _fp]),
+    "ktt_coulomb3d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _vp, _fp]),
+    "ktt_coulomb2d": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _c.c_char_p, _vp, _fp]),
+    "ktt_coulomb2d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _vp, _fp]),
+    "ktt_coulomb3d_iterative": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_char_p, _vp, _fp, _fp]),
+    "ktt_coulomb3d_slices": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_char_p, _vp, _fp]),
+    "ktt_coulomb_launch_shape": (_c.c_int, [_c.c_int, _c.c_int, _c.c_char_p, _uint3, _uint3]),
+    "ktt_coulomb3d_tune": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_char_p,
+                                      _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_double), _c.POINTER(_c.c_int),
+                                      _c.POINTER(_c.c_int), _vp]),
+}
+
+
+class KttHipError(RuntimeError):
+    """A C entry point returned an error."""
+
+
+_library = None
+
+
+def library() -> ctypes.CDLL:
+    global _library
+    if _library is None:
+        if not LIBRARY_PATH.exists():
+            raise ImportError(f"{LIBRARY_PATH} is not built; run `python -m ktt_amd.build` (or __graft_entry__.build())")
+        handle = ctypes.CDLL(str(LIBRARY_PATH))
+        for name, (restype, argtypes) in SIGNATURES.items():
+            function = getattr(handle, name)
+            function.restype = restype
+            function.argtypes = argtypes
+        _library = handle
+    return _library
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        message = library().ktt_hip_last_error().decode(errors="replace")
+        raise KttHipError(f"{what} failed: {message}")
+
+
+def _config(config) -> bytes | None:
+    if config is None:
+        return None
+    if isinstance(config, dict):
+        config = ",".join(f"{key}={value}" for key, value in config.items())
+    return config.encode()
+
+
+def device_count() -> int:
+    count = ctypes.c_int(0)
+    _check(library().ktt_hip_device_count(ctypes.byref(count)), "ktt_hip_device_count")
+    return count.value
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(256)
+    arch = ctypes.create_string_buffer(64)
+    units = ctypes.c_int(0)
+    memory = ctypes.c_size_t(0)
+    _check(library().ktt_hip_device_info(name, 256, arch, 64, ctypes.byref(units), ctypes.byref(memory)), "ktt_hip_device_info")
+    return {"name": name.value.decode(), "arch": arch.value.decode(), "compute_units": units.value, "memory": memory.value}
+
+
+def compile_check(source: str, kernel: str, config=None) -> str:
+    log = ctypes.create_string_buffer(65536)
+    _check(library().ktt_hip_compile_check(source.encode(), kernel.encode(), _config(config), log, len(log)),
+           f"compile {source}:{kernel}")
+    return log.value.decode(errors="replace")
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python; freed on close() or garbage collection."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        pointer = ctypes.c_void_p()
+        _check(library().ktt_hip_malloc(ctypes.byref(pointer), max(self.nbytes, 4)), "ktt_hip_malloc")
+        self.pointer = pointer
+
+    @classmethod
+    def from_array(cls, array: np.ndarray) -> "DeviceBuffer":
+        array = np.ascontiguousarray(array)
+        buffer = cls(array.nbytes)
+        buffer.upload(array)
+        return buffer
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self.pointer
+
+    def upload(self, array: np.ndarray) -> None:
+        array = np.ascontiguousarray(array)
+        if array.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if array.nbytes:
+            _check(library().ktt_hip_memcpy_htod(self.pointer, array.ctypes.data_as(ctypes.c_void_p), array.nbytes), "upload")
+
+    def download(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more bytes than the buffer holds")
+        if out.nbytes:
+            _check(library().ktt_hip_memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), self.pointer, out.nbytes), "download")
+        return out
+
+    def zero(self, stream=None) -> None:
+        _check(library().ktt_hip_zero(self.pointer, self.nbytes - self.nbytes % 4, stream), "ktt_hip_zero")
+
+    def close(self) -> None:
+        if self.pointer is not None and self.pointer.value:
+            library().ktt_hip_free(self.pointer)
+        self.pointer = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    _check(library().ktt_hip_synchronize(), "ktt_hip_synchronize")
+
+
+class Event:
+    def __init__(self):
+        handle = ctypes.c_void_p()
+        _check(library().ktt_hip_event_create(ctypes.byref(handle)), "ktt_hip_event_create")
+        self.handle = handle
+
+    def record(self, stream=None) -> None:
+        _check(library().ktt_hip_event_record(self.handle, stream), "ktt_hip_event_record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        value = ctypes.c_float(0.0)
+        _check(library().ktt_hip_event_elapsed_ms(self.handle, end.handle, ctypes.byref(value)), "ktt_hip_event_elapsed_ms")
+        return value.value
+
+    def __del__(self):
+        try:
+            library().ktt_hip_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _timing(timed: bool):
+    return (ctypes.c_float(0.0), True) if timed else (None, False)
+
+
+def coulomb3d(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, energy: DeviceBuffer, form: int = PLAIN,
+              config=None, stream=None, timed: bool = True) -> float | None:
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb3d(atoms.ptr, n_atoms, spacing, grid, energy.ptr, form, _config(config), stream,
+                                   ctypes.byref(ms) if timed else None), "ktt_coulomb3d")
+    return ms.value if timed else None
+
+
+def coulomb3d_reference(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, energy: DeviceBuffer, stream=None,
+                        timed: bool = True) -> float | None:
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb3d_reference(atoms.ptr, n_atoms, spacing, grid, energy.ptr, stream,
+                                             ctypes.byref(ms) if timed else None), "ktt_coulomb3d_reference")
+    return ms.value if timed else None
+
+
+def coulomb2d(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid_x: int, grid_y: int, energy: DeviceBuffer, config=None,
+              stream=None, timed: bool = True) -> float | None:
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb2d(atoms.ptr, n_atoms, spacing, grid_x, grid_y, energy.ptr, _config(config), stream,
+                                   ctypes.byref(ms) if timed else None), "ktt_coulomb2d")
+    return ms.value if timed else None
+
+
+def coulomb2d_reference(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid_x: int, grid_y: int, energy: DeviceBuffer,
+                        stream=None, timed: bool = True) -> float | None:
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb2d_reference(atoms.ptr, n_atoms, spacing, grid_x, grid_y, energy.ptr, stream,
+                                             ctypes.byref(ms) if timed else None), "ktt_coulomb2d_reference")
+    return ms.value if timed else None
+
+
+def coulomb3d_iterative(host_atoms: np.ndarray, spacing: float, grid: int, energy: DeviceBuffer, config=None,
+                        stream=None) -> tuple[float, float]:
+    host_atoms = np.ascontiguousarray(host_atoms, dtype=np.float32)
+    kernel_ms = ctypes.c_float(0.0)
+    wall_ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb3d_iterative(host_atoms.ctypes.data_as(ctypes.c_void_p), host_atoms.shape[0], spacing, grid,
+                                             energy.ptr, _config(config), stream, ctypes.byref(kernel_ms),
+                                             ctypes.byref(wall_ms)), "ktt_coulomb3d_iterative")
+    return kernel_ms.value, wall_ms.value
+
+
+def coulomb3d_slices(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, energy: DeviceBuffer, config=None,
+                     stream=None, timed: bool = True) -> float | None:
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb3d_slices(atoms.ptr, n_atoms, spacing, grid, energy.ptr, _config(config), stream,
+                                          ctypes.byref(ms) if timed else None), "ktt_coulomb3d_slices")
+    return ms.value if timed else None
+
+
+def launch_shape(kind: int, grid: int, config=None) -> tuple[tuple[int, int, int], tuple[int, int, int]]:
+    g = _uint3()
+    b = _uint3()
+    _check(library().ktt_coulomb_launch_shape(kind, grid, _config(config), g, b), "ktt_coulomb_launch_shape")
+    return tuple(g), tuple(b)
+
+
+def coulomb3d_tune(host_atoms: np.ndarray, spacing: float, grid: int, searcher: int = SEARCHER_DETERMINISTIC,
+                   max_configurations: int = 0, tolerance: float = 0.01, results_file: str | None = None,
+                   want_grid: bool = False) -> dict:
+    host_atoms = np.ascontiguousarray(host_atoms, dtype=np.float32)
+    best = ctypes.create_string_buffer(1024)
+    best_ms = ctypes.c_double(0.0)
+    valid = ctypes.c_int(0)
+    tested = ctypes.c_int(0)
+    output = np.zeros((grid, grid, grid), dtype=np.float32) if want_grid else None
+    _check(library().ktt_coulomb3d_tune(host_atoms.ctypes.data_as(ctypes.c_void_p), host_atoms.shape[0], spacing, grid, searcher,
+                                        max_configurations, tolerance, results_file.encode() if results_file else None, best,
+                                        len(best), ctypes.byref(best_ms), ctypes.byref(valid), ctypes.byref(tested),
+                                        output.ctypes.data_as(ctypes.c_void_p) if output is not None else None),
+           "ktt_coulomb3d_tune")
+    return {"best_config": best.value.decode(), "best_ms": best_ms.value, "valid": valid.value, "tested": tested.value,
+            "grid": output}

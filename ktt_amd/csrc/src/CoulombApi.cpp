@@ -1,0 +1,879 @@
+// C entry points of include/ktt_hip.h, sections 1-3: library/device queries, memory and timing
+// plumbing, and the direct Coulomb launches on device buffers.
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+#include <KttException.h>
+
+#include "../../../include/ktt_hip.h"
+#include "Common.h"
+#include "CoulombRuntime.h"
+#include "HipEngine.h"
+#include "Kernel.h"
+
+namespace ktt
+{
+
+static thread_local std::string g_LastError;
+
+int SetLastError(const std::string& message)
+{
+    g_LastError = message;
+    return -1;
+}
+
+// ---------------------------------------------------------------- CoulombConfig
+
+bool CoulombConfig::Has(const std::string& name) const
+{
+    return std::any_of(values.cbegin(), values.cend(), [&name](const auto& entry) { return entry.first == name; });
+}
+
+long CoulombConfig::Get(const std::string& name, const long fallback) const
+{
+    for (const auto& [key, value] : values)
+    {
+        if (key == name)
+        {
+            char* end = nullptr;
+            const long result = std::strtol(value.c_str(), &end, 10);
+            if (end == value.c_str() || *end != '\0')
+            {
+                throw KttException("Configuration value " + name + "=" + value + " is not an integer");
+            }
+            return result;
+        }
+    }
+    return fallback;
+}
+
+void CoulombConfig::Set(const std::string& name, const std::string& value)
+{
+    for (auto& [key, entry] : values)
+    {
+        if (key == name)
+        {
+            entry = value;
+            return;
+        }
+    }
+    values.emplace_back(name, value);
+}
+
+std::vector<std::string> CoulombConfig::Defines(const std::vector<std::string>& launchOnly) const
+{
+    std::vector<std::string> result;
+    for (const auto& [key, value] : values)
+    {
+        if (std::find(launchOnly.cbegin(), launchOnly.cend(), key) == launchOnly.cend())
+        {
+            result.push_back("-D" + key + "=" + value);
+        }
+    }
+    return result;
+}
+
+std::string CoulombConfig::ToString() const
+{
+    std::string result;
+    for (const auto& [key, value] : values)
+    {
+        if (!result.empty())
+        {
+            result += ",";
+        }
+        result += key + "=" + value;
+    }
+    return result;
+}
+
+CoulombConfig CoulombConfig::Parse(const char* text, const CoulombConfig& defaults)
+{
+    CoulombConfig result = defaults;
+    if (text == nullptr)
+    {
+        return result;
+    }
+    std::stringstream stream(text);
+    std::string item;
+    while (std::getline(stream, item, ','))
+    {
+        item.erase(std::remove_if(item.begin(), item.end(), [](const char c) { return c == ' ' || c == '\t'; }), item.end());
+        if (item.empty())
+        {
+            continue;
+        }
+        const size_t equals = item.find('=');
+        if (equals == std::string::npos || equals == 0 || equals + 1 == item.size())
+        {
+            throw KttException("Invalid configuration item '" + item + "', expected NAME=VALUE");
+        }
+        const std::string name = item.substr(0, equals);
+        for (const char c : name)
+        {
+            if (!(std::isalnum(static_cast<unsigned char>(c)) || c == '_'))
+            {
+                throw KttException("Invalid configuration name '" + name + "'");
+            }
+        }
+        result.Set(name, item.substr(equals + 1));
+    }
+    return result;
+}
+
+// ---------------------------------------------------------------- CoulombRuntime
+
+static std::string LibraryDirectory()
+{
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&LibraryDirectory), &info) != 0 && info.dli_fname != nullptr)
+    {
+        std::string path = info.dli_fname;
+        const size_t slash = path.find_last_of('/');
+        if (slash != std::string::npos)
+        {
+            return path.substr(0, slash);
+        }
+    }
+    return ".";
+}
+
+CoulombRuntime::CoulombRuntime() :
+    m_DeviceChecked(false)
+{
+    if (const char* directory = std::getenv("KTT_HIP_KERNEL_DIR"))
+    {
+        m_KernelDirectory = directory;
+    }
+    else
+    {
+        m_KernelDirectory = LibraryDirectory() + "/kernels";
+    }
+    m_Architecture = "gfx950";
+}
+
+CoulombRuntime& CoulombRuntime::Get()
+{
+    static CoulombRuntime runtime;
+    return runtime;
+}
+
+void CoulombRuntime::EnsureDevice()
+{
+    std::lock_guard<std::mutex> lock(m_Mutex);
+    if (m_DeviceChecked)
+    {
+        return;
+    }
+    int count = 0;
+    const hipError_t status = hipGetDeviceCount(&count);
+    if (status != hipSuccess || count == 0)
+    {
+        throw KttException("No HIP device is available (hipGetDeviceCount: " + std::string(hipGetErrorString(status)) + ")");
+    }
+    int device = 0;
+    CheckHipError(hipGetDevice(&device), "hipGetDevice");
+    m_Architecture = HipCompiler::DeviceArchitecture(device);
+    m_DeviceChecked = true;
+}
+
+std::shared_ptr<HipKernel> CoulombRuntime::GetKernel(const std::string& file, const std::string& kernel,
+    const std::vector<std::string>& defines)
+{
+    EnsureDevice();
+    std::string key = file + "|" + kernel;
+    for (const auto& define : defines)
+    {
+        key += "|" + define;
+    }
+    {
+        std::lock_guard<std::mutex> lock(m_Mutex);
+        auto iterator = m_Kernels.find(key);
+        if (iterator != m_Kernels.end())
+        {
+            return iterator->second;
+        }
+    }
+
+    const std::string path = m_KernelDirectory + "/" + file;
+    const std::string source = LoadFileToString(path);
+    std::vector<std::string> options = HipCompiler::DefaultOptions(m_Architecture);
+    options.push_back("-I" + m_KernelDirectory);
+    for (const auto& define : defines)
+    {
+        options.push_back(define);
+    }
+    Timer timer;
+    timer.Start();
+    std::string lowered;
+    const auto code = HipCompiler::Compile(source, file, kernel, options, lowered);
+    timer.Stop();
+    auto compiled = std::make_shared<HipKernel>(kernel, code, lowered, timer.GetElapsedTime());
+
+    std::lock_guard<std::mutex> lock(m_Mutex);
+    m_Kernels[key] = compiled;
+    return compiled;
+}
+
+void* CoulombRuntime::GetScratch(const std::string& name, const size_t bytes)
+{
+    std::lock_guard<std::mutex> lock(m_Mutex);
+    auto& entry = m_Scratch[name];
+    if (entry.second < bytes)
+    {
+        if (entry.first != nullptr)
+        {
+            CheckHipError(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            IgnoreHipError(hipFree(entry.first));
+            entry = {nullptr, 0};
+        }
+        CheckHipError(hipMalloc(&entry.first, bytes), "hipMalloc(scratch " + name + ")");
+        CheckHipError(hipMemset(entry.first, 0, bytes), "hipMemset(scratch " + name + ")");
+        CheckHipError(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        entry.second = bytes;
+    }
+    return entry.first;
+}
+
+CoulombShape CoulombRuntime::RowShape(const CoulombConfig& config, const long spanX, const long rows, const long atomBlocks)
+{
+    const long wgx = config.Get("WORK_GROUP_SIZE_X", 64);
+    const long wgy = config.Get("WORK_GROUP_SIZE_Y", 4);
+    const long px = config.Get("POINTS_X", 4);
+    const long pr = config.Get("ROWS_PER_THREAD", 16);
+    if (wgx <= 0 || wgy <= 0 || px <= 0 || pr <= 0 || atomBlocks <= 0)
+    {
+        throw KttException("Workgroup shape, points per thread and atom blocks must be positive");
+    }
+    if (wgx * wgy > 1024)
+    {
+        throw KttException("Workgroup of " + std::to_string(wgx * wgy) + " threads exceeds the device limit of 1024",
+            ExceptionReason::DeviceLimitsExceeded);
+    }
+    const long threads = wgx * wgy;
+    const long rowsPerGroup = wgy * pr;
+    if (threads % rowsPerGroup != 0 && rowsPerGroup % threads != 0)
+    {
+        throw KttException("Rows per workgroup (" + std::to_string(rowsPerGroup) + ") and threads per workgroup ("
+            + std::to_string(threads) + ") must divide one another");
+    }
+    CoulombShape shape;
+    const long spanPerGroup = wgx * px;
+    const long groupsX = (spanX + spanPerGroup - 1) / spanPerGroup;
+    const long groupsY = (rows + rowsPerGroup - 1) / rowsPerGroup;
+    if (groupsX > 0x7fffffffL || groupsY > 65535L * 65535L || atomBlocks > 65535L)
+    {
+        throw KttException("Launch grid exceeds the device limits", ExceptionReason::DeviceLimitsExceeded);
+    }
+    if (groupsY > 65535L)
+    {
+        throw KttException("Launch grid of " + std::to_string(groupsY) + " row groups exceeds 65535; raise ROWS_PER_THREAD or "
+            "WORK_GROUP_SIZE_Y", ExceptionReason::DeviceLimitsExceeded);
+    }
+    shape.grid[0] = static_cast<unsigned int>(groupsX);
+    shape.grid[1] = static_cast<unsigned int>(groupsY);
+    shape.grid[2] = static_cast<unsigned int>(atomBlocks);
+    shape.block[0] = static_cast<unsigned int>(wgx);
+    shape.block[1] = static_cast<unsigned int>(wgy);
+    shape.block[2] = 1;
+    shape.threads = threads;
+    shape.rowsPerGroup = rowsPerGroup;
+    shape.pointsPerThread = px * pr;
+    return shape;
+}
+
+void CoulombRuntime::Launch(const HipKernel& kernel, const unsigned int grid[3], const unsigned int block[3], void** arguments,
+    hipStream_t stream, float* kernelMs)
+{
+    if (grid[0] == 0 || grid[1] == 0 || grid[2] == 0)
+    {
+        if (kernelMs != nullptr)
+        {
+            *kernelMs = 0.0f;
+        }
+        return;
+    }
+    hipEvent_t start = nullptr;
+    hipEvent_t end = nullptr;
+    if (kernelMs != nullptr)
+    {
+        CheckHipError(hipEventCreate(&start), "hipEventCreate");
+        CheckHipError(hipEventCreate(&end), "hipEventCreate");
+        CheckHipError(hipEventRecord(start, stream), "hipEventRecord");
+    }
+    const hipError_t status = hipModuleLaunchKernel(kernel.GetFunction(), grid[0], grid[1], grid[2], block[0], block[1], block[2], 0,
+        stream, arguments, nullptr);
+    if (status != hipSuccess)
+    {
+        if (start != nullptr)
+        {
+            IgnoreHipError(hipEventDestroy(start));
+            IgnoreHipError(hipEventDestroy(end));
+        }
+        CheckHipError(status, "hipModuleLaunchKernel(" + kernel.GetName() + ")");
+    }
+    if (kernelMs != nullptr)
+    {
+        CheckHipError(hipEventRecord(end, stream), "hipEventRecord");
+        const hipError_t sync = hipEventSynchronize(end);
+        float milliseconds = 0.0f;
+        if (sync == hipSuccess)
+        {
+            IgnoreHipError(hipEventElapsedTime(&milliseconds, start, end));
+        }
+        IgnoreHipError(hipEventDestroy(start));
+        IgnoreHipError(hipEventDestroy(end));
+        CheckHipError(sync, "hipEventSynchronize(" + kernel.GetName() + ")");
+        *kernelMs = milliseconds;
+    }
+}
+
+// Tuned defaults (gfx950).
+static CoulombConfig Defaults3d()
+{
+    CoulombConfig config;
+    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "4"}, {"ROWS_PER_THREAD", "16"},
+        {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
+    return config;
+}
+
+static CoulombConfig DefaultsSlice()
+{
+    CoulombConfig config;
+    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "2"}, {"ROWS_PER_THREAD", "8"},
+        {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "16"}};
+    return config;
+}
+
+static const std::vector<std::string> kLaunchOnly = {"ATOM_BLOCKS"};
+
+static void CheckAtoms(const float* atoms, const int atomCount)
+{
+    if (atomCount < 0)
+    {
+        throw KttException("Number of atoms must not be negative");
+    }
+    if (atomCount > 0 && atoms == nullptr)
+    {
+        throw KttException("Atom buffer is null");
+    }
+}
+
+static void CheckGrid(const float* grid, const long points)
+{
+    if (points <= 0)
+    {
+        throw KttException("Grid must have at least one point");
+    }
+    if (grid == nullptr)
+    {
+        throw KttException("Energy grid buffer is null");
+    }
+}
+
+static long EffectiveAtomBlocks(const CoulombConfig& config, const int atomCount)
+{
+    const long tile = std::max<long>(1, config.Get("ATOM_TILE", 64));
+    const long tiles = std::max<long>(1, (atomCount + tile - 1) / tile);
+    return std::max<long>(1, std::min(config.Get("ATOM_BLOCKS", 1), tiles));
+}
+
+} // namespace ktt
+
+using namespace ktt;
+
+#define KTT_C_TRY try {
+#define KTT_C_CATCH } catch (const std::exception& exception) { return SetLastError(exception.what()); } catch (...) { return SetLastError("unknown error"); }
+
+extern "C" {
+
+int ktt_hip_abi_version(void)
+{
+    return 100;
+}
+
+const char* ktt_hip_last_error(void)
+{
+    return g_LastError.c_str();
+}
+
+int ktt_hip_device_count(int* count)
+{
+    KTT_C_TRY
+    if (count == nullptr)
+    {
+        return SetLastError("count is null");
+    }
+    *count = 0;
+    if (hipGetDeviceCount(count) != hipSuccess)
+    {
+        *count = 0;
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_device_info(char* name, size_t nameLength, char* architecture, size_t architectureLength, int* computeUnits,
+    size_t* globalMemory)
+{
+    KTT_C_TRY
+    CoulombRuntime::Get().EnsureDevice();
+    int device = 0;
+    CheckHipError(hipGetDevice(&device), "hipGetDevice");
+    hipDeviceProp_t properties;
+    CheckHipError(hipGetDeviceProperties(&properties, device), "hipGetDeviceProperties");
+    if (name != nullptr && nameLength > 0)
+    {
+        std::snprintf(name, nameLength, "%s", properties.name);
+    }
+    if (architecture != nullptr && architectureLength > 0)
+    {
+        std::snprintf(architecture, architectureLength, "%s", HipCompiler::DeviceArchitecture(device).c_str());
+    }
+    if (computeUnits != nullptr)
+    {
+        *computeUnits = properties.multiProcessorCount;
+    }
+    if (globalMemory != nullptr)
+    {
+        *globalMemory = properties.totalGlobalMem;
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
+const char* ktt_hip_kernel_directory(void)
+{
+    return CoulombRuntime::Get().GetKernelDirectory().c_str();
+}
+
+int ktt_hip_compile_check(const char* sourceFile, const char* kernelName, const char* config, char* log, size_t logLength)
+{
+    KTT_C_TRY
+    if (sourceFile == nullptr || kernelName == nullptr)
+    {
+        return SetLastError("source file and kernel name are required");
+    }
+    std::string path = sourceFile;
+    if (path.find('/') == std::string::npos)
+    {
+        path = CoulombRuntime::Get().GetKernelDirectory() + "/" + path;
+    }
+    const std::string source = LoadFileToString(path);
+    const size_t slash = path.find_last_of('/');
+    std::vector<std::string> options = HipCompiler::DefaultOptions("gfx950");
+    options.push_back("-I" + (slash == std::string::npos ? std::string(".") : path.substr(0, slash)));
+    for (const auto& define : CoulombConfig::Parse(config, CoulombConfig()).Defines(kLaunchOnly))
+    {
+        options.push_back(define);
+    }
+    std::string lowered;
+    std::string compilerLog;
+    try
+    {
+        HipCompiler::Compile(source, path, kernelName, options, lowered, &compilerLog);
+    }
+    catch (const KttException& exception)
+    {
+        if (log != nullptr && logLength > 0)
+        {
+            std::snprintf(log, logLength, "%s", exception.what());
+        }
+        throw;
+    }
+    if (log != nullptr && logLength > 0)
+    {
+        std::snprintf(log, logLength, "%s", compilerLog.c_str());
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_malloc(void** pointer, size_t bytes)
+{
+    KTT_C_TRY
+    if (pointer == nullptr)
+    {
+        return SetLastError("pointer is null");
+    }
+    CoulombRuntime::Get().EnsureDevice();
+    CheckHipError(hipMalloc(pointer, bytes), "hipMalloc");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_free(void* pointer)
+{
+    KTT_C_TRY
+    CheckHipError(hipFree(pointer), "hipFree");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_memcpy_htod(void* destination, const void* source, size_t bytes)
+{
+    KTT_C_TRY
+    CheckHipError(hipMemcpy(destination, source, bytes, hipMemcpyHostToDevice), "hipMemcpy(HtoD)");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_memcpy_dtoh(void* destination, const void* source, size_t bytes)
+{
+    KTT_C_TRY
+    CheckHipError(hipMemcpy(destination, source, bytes, hipMemcpyDeviceToHost), "hipMemcpy(DtoH)");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_zero(void* pointer, size_t bytes, void* stream)
+{
+    KTT_C_TRY
+    if (bytes == 0)
+    {
+        return 0;
+    }
+    if (pointer == nullptr)
+    {
+        return SetLastError("pointer is null");
+    }
+    if (bytes % 4 != 0 || reinterpret_cast<uintptr_t>(pointer) % 4 != 0)
+    {
+        return SetLastError("ktt_hip_zero needs a 4-byte aligned pointer and a multiple of 4 bytes");
+    }
+    auto& runtime = CoulombRuntime::Get();
+    const bool wide = bytes % 16 == 0 && reinterpret_cast<uintptr_t>(pointer) % 16 == 0;
+    auto kernel = runtime.GetKernel("coulomb_util.hip", wide ? "kttZeroFill16" : "kttZeroFill4", {});
+    unsigned long long count = wide ? bytes / 16 : bytes / 4;
+    void* arguments[] = {&pointer, &count};
+    const unsigned long long groups = std::min<unsigned long long>((count + 255) / 256, 8192);
+    const unsigned int grid[3] = {static_cast<unsigned int>(groups), 1, 1};
+    const unsigned int block[3] = {256, 1, 1};
+    CoulombRuntime::Launch(*kernel, grid, block, arguments, static_cast<hipStream_t>(stream), nullptr);
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_synchronize(void)
+{
+    KTT_C_TRY
+    CheckHipError(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_stream_create(void** stream)
+{
+    KTT_C_TRY
+    CoulombRuntime::Get().EnsureDevice();
+    CheckHipError(hipStreamCreate(reinterpret_cast<hipStream_t*>(stream)), "hipStreamCreate");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_stream_destroy(void* stream)
+{
+    KTT_C_TRY
+    CheckHipError(hipStreamDestroy(static_cast<hipStream_t>(stream)), "hipStreamDestroy");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_event_create(void** event)
+{
+    KTT_C_TRY
+    CoulombRuntime::Get().EnsureDevice();
+    CheckHipError(hipEventCreate(reinterpret_cast<hipEvent_t*>(event)), "hipEventCreate");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_event_destroy(void* event)
+{
+    KTT_C_TRY
+    CheckHipError(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_event_record(void* event, void* stream)
+{
+    KTT_C_TRY
+    CheckHipError(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)), "hipEventRecord");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_hip_event_elapsed_ms(void* start, void* end, float* milliseconds)
+{
+    KTT_C_TRY
+    CheckHipError(hipEventSynchronize(static_cast<hipEvent_t>(end)), "hipEventSynchronize");
+    CheckHipError(hipEventElapsedTime(milliseconds, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)),
+        "hipEventElapsedTime");
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb_launch_shape(int kind, int gridSize, const char* config, unsigned int grid[3], unsigned int block[3])
+{
+    KTT_C_TRY
+    if (gridSize <= 0 || grid == nullptr || block == nullptr)
+    {
+        return SetLastError("invalid arguments");
+    }
+    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d() : DefaultsSlice());
+    const long rows = kind == 3 ? static_cast<long>(gridSize) * gridSize : gridSize;
+    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, rows, parsed.Get("ATOM_BLOCKS", 1));
+    for (int i = 0; i < 3; ++i)
+    {
+        grid[i] = shape.grid[i];
+        block[i] = shape.block[i];
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize, float* energyGrid, int form, const char* config,
+    void* stream, float* kernelMs)
+{
+    KTT_C_TRY
+    if (gridSize <= 0 || static_cast<long>(gridSize) * gridSize > 0x7fffffffL)
+    {
+        return SetLastError("grid size must be in [1, 46340]");
+    }
+    CheckAtoms(atoms, atomCount);
+    CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
+    if (form != KTT_COULOMB_PLAIN && form != KTT_COULOMB_COMPENSATED)
+    {
+        return SetLastError("unknown form " + std::to_string(form));
+    }
+    auto& runtime = CoulombRuntime::Get();
+    const CoulombConfig parsed = CoulombConfig::Parse(config, Defaults3d());
+    const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
+    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks);
+    const std::vector<std::string> defines = parsed.Defines(kLaunchOnly);
+    const float* unused = nullptr;
+    hipStream_t hipStream = static_cast<hipStream_t>(stream);
+
+    if (form == KTT_COULOMB_PLAIN)
+    {
+        auto kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSum", defines);
+        void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid};
+        CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, kernelMs);
+        return 0;
+    }
+
+    auto kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSumCompensated", defines);
+    const size_t tiles = static_cast<size_t>(shape.grid[0]) * shape.grid[1];
+    void* partials = nullptr;
+    if (atomBlocks > 1)
+    {
+        const size_t bytes = static_cast<size_t>(atomBlocks) * tiles * static_cast<size_t>(shape.threads)
+            * static_cast<size_t>(shape.pointsPerThread) * 8;
+        partials = runtime.GetScratch("coulomb3d.partials", bytes);
+    }
+    void* counters = runtime.GetScratch("coulomb3d.counters", tiles * sizeof(unsigned int));
+    void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid, &partials, &counters};
+    CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb3d_reference(const float* atoms, int atomCount, float spacing, int gridSize, float* energyGrid, void* stream,
+    float* kernelMs)
+{
+    KTT_C_TRY
+    if (gridSize <= 0 || static_cast<long>(gridSize) * gridSize > 0x7fffffffL)
+    {
+        return SetLastError("grid size must be in [1, 46340]");
+    }
+    CheckAtoms(atoms, atomCount);
+    CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
+    auto kernel = CoulombRuntime::Get().GetKernel("CoulombSum3dReference.hip", "directCoulombSumReference", {});
+    const unsigned int grid[3] = {static_cast<unsigned int>((gridSize + 15) / 16), static_cast<unsigned int>((gridSize + 15) / 16),
+        static_cast<unsigned int>(gridSize)};
+    const unsigned int block[3] = {16, 16, 1};
+    void* arguments[] = {&atoms, &atomCount, &spacing, &gridSize, &energyGrid};
+    CoulombRuntime::Launch(*kernel, grid, block, arguments, static_cast<hipStream_t>(stream), kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb2d(const float* atoms, int atomCount, float spacing, int gridX, int gridY, float* energyGrid, const char* config,
+    void* stream, float* kernelMs)
+{
+    KTT_C_TRY
+    CheckAtoms(atoms, atomCount);
+    if (gridX <= 0 || gridY <= 0)
+    {
+        return SetLastError("grid sizes must be positive");
+    }
+    CheckGrid(energyGrid, static_cast<long>(gridX) * gridY);
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice());
+    const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
+    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridX, gridY, atomBlocks);
+    // The slice kernel derives the row length and row count from the launch (as the reference
+    // 2D kernel does), so the grid must be covered exactly.
+    const long spanX = static_cast<long>(shape.block[0]) * parsed.Get("POINTS_X", 4);
+    if (gridX % spanX != 0 || gridY % shape.rowsPerGroup != 0)
+    {
+        return SetLastError("grid " + std::to_string(gridX) + "x" + std::to_string(gridY) + " is not a multiple of the workgroup span "
+            + std::to_string(spanX) + "x" + std::to_string(shape.rowsPerGroup));
+    }
+    auto kernel = CoulombRuntime::Get().GetKernel("CoulombSum2d.hip", "directCoulombSum", parsed.Defines(kLaunchOnly));
+    const float* unused = nullptr;
+    void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &energyGrid};
+    CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, static_cast<hipStream_t>(stream), kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb2d_reference(const float* atoms, int atomCount, float spacing, int gridX, int gridY, float* energyGrid, void* stream,
+    float* kernelMs)
+{
+    KTT_C_TRY
+    CheckAtoms(atoms, atomCount);
+    if (gridX <= 0 || gridY <= 0 || gridX % 16 != 0 || gridY % 16 != 0)
+    {
+        return SetLastError("the 2D reference kernel needs grid sizes that are positive multiples of 16");
+    }
+    CheckGrid(energyGrid, static_cast<long>(gridX) * gridY);
+    auto kernel = CoulombRuntime::Get().GetKernel("CoulombSum2dReference.hip", "directCoulombSumReference", {});
+    const unsigned int grid[3] = {static_cast<unsigned int>(gridX / 16), static_cast<unsigned int>(gridY / 16), 1};
+    const unsigned int block[3] = {16, 16, 1};
+    void* arguments[] = {&atoms, &atomCount, &spacing, &energyGrid};
+    CoulombRuntime::Launch(*kernel, grid, block, arguments, static_cast<hipStream_t>(stream), kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
+static void CheckSliceGrid(const CoulombShape& shape, const CoulombConfig& parsed, const int gridSize)
+{
+    const long spanX = static_cast<long>(shape.block[0]) * parsed.Get("POINTS_X", 4);
+    if (gridSize % spanX != 0 || gridSize % shape.rowsPerGroup != 0)
+    {
+        throw KttException("grid " + std::to_string(gridSize) + " is not a multiple of the workgroup span " + std::to_string(spanX) + "x"
+            + std::to_string(shape.rowsPerGroup));
+    }
+}
+
+int ktt_coulomb3d_iterative(const float* hostAtoms, int atomCount, float spacing, int gridSize, float* energyGrid, const char* config,
+    void* stream, float* kernelMs, float* wallMs)
+{
+    KTT_C_TRY
+    CheckAtoms(hostAtoms, atomCount);
+    if (gridSize <= 0 || static_cast<long>(gridSize) * gridSize > 0x7fffffffL)
+    {
+        return SetLastError("grid size must be in [1, 46340]");
+    }
+    CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
+    auto& runtime = CoulombRuntime::Get();
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice());
+    const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
+    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, gridSize, atomBlocks);
+    CheckSliceGrid(shape, parsed, gridSize);
+    auto kernel = runtime.GetKernel("CoulombSum3dIterative.hip", "directCoulombSum", parsed.Defines(kLaunchOnly));
+    hipStream_t hipStream = static_cast<hipStream_t>(stream);
+
+    const size_t bytes = static_cast<size_t>(std::max(atomCount, 1)) * 4 * sizeof(float);
+    float* deviceAtoms = static_cast<float*>(runtime.GetScratch("iterative.atoms", bytes));
+    void* pinned = nullptr;
+    CheckHipError(hipHostMalloc(&pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
+    float* staging = static_cast<float*>(pinned);
+    std::memcpy(staging, hostAtoms, static_cast<size_t>(atomCount) * 4 * sizeof(float));
+
+    const auto begin = std::chrono::steady_clock::now();
+    float kernelTotal = 0.0f;
+    try
+    {
+        const float* unused = nullptr;
+        const float* atomsArgument = deviceAtoms;
+        for (int z = 0; z < gridSize; ++z)
+        {
+            // Host precompute of dz^2 in FP32 (CoulombSum3dIterative.cpp:160-168).
+            const float coordinate = spacing * static_cast<float>(z);
+            for (int i = 0; i < atomCount; ++i)
+            {
+                const float d = coordinate - hostAtoms[4 * i + 2];
+                staging[4 * i + 2] = d * d;
+            }
+            CheckHipError(hipMemcpyAsync(deviceAtoms, staging, static_cast<size_t>(atomCount) * 4 * sizeof(float), hipMemcpyHostToDevice,
+                hipStream), "hipMemcpyAsync");
+            CheckHipError(hipStreamSynchronize(hipStream), "hipStreamSynchronize");
+            int zIndex = z;
+            void* arguments[] = {&atomsArgument, &unused, &unused, &unused, &unused, &atomCount, &spacing, &zIndex, &energyGrid};
+            float sliceMs = 0.0f;
+            CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, &sliceMs);
+            kernelTotal += sliceMs;
+        }
+    }
+    catch (...)
+    {
+        IgnoreHipError(hipHostFree(pinned));
+        throw;
+    }
+    const auto finish = std::chrono::steady_clock::now();
+    IgnoreHipError(hipHostFree(pinned));
+    if (kernelMs != nullptr)
+    {
+        *kernelMs = kernelTotal;
+    }
+    if (wallMs != nullptr)
+    {
+        *wallMs = std::chrono::duration<float, std::milli>(finish - begin).count();
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb3d_slices(const float* atoms, int atomCount, float spacing, int gridSize, float* energyGrid, const char* config,
+    void* stream, float* kernelMs)
+{
+    KTT_C_TRY
+    CheckAtoms(atoms, atomCount);
+    if (gridSize <= 0 || static_cast<long>(gridSize) * gridSize > 0x7fffffffL)
+    {
+        return SetLastError("grid size must be in [1, 46340]");
+    }
+    CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
+    auto& runtime = CoulombRuntime::Get();
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice());
+    const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
+    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, gridSize, atomBlocks);
+    CheckSliceGrid(shape, parsed, gridSize);
+    auto kernel = runtime.GetKernel("CoulombSum3dIterative.hip", "directCoulombSumSlice", parsed.Defines(kLaunchOnly));
+    hipStream_t hipStream = static_cast<hipStream_t>(stream);
+
+    hipEvent_t start = nullptr;
+    hipEvent_t end = nullptr;
+    if (kernelMs != nullptr)
+    {
+        CheckHipError(hipEventCreate(&start), "hipEventCreate");
+        CheckHipError(hipEventCreate(&end), "hipEventCreate");
+        CheckHipError(hipEventRecord(start, hipStream), "hipEventRecord");
+    }
+    const float* unused = nullptr;
+    for (int z = 0; z < gridSize; ++z)
+    {
+        int zIndex = z;
+        void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &zIndex, &energyGrid};
+        CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, nullptr);
+    }
+    if (kernelMs != nullptr)
+    {
+        CheckHipError(hipEventRecord(end, hipStream), "hipEventRecord");
+        CheckHipError(hipEventSynchronize(end), "hipEventSynchronize");
+        IgnoreHipError(hipEventElapsedTime(kernelMs, start, end));
+        IgnoreHipError(hipEventDestroy(start));
+        IgnoreHipError(hipEventDestroy(end));
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
+} // extern "C"

@@ -1,0 +1,72 @@
+// Owner of the managers, the compute engine and the runners (Source/TunerCore.cpp:21-35).
+// The engine factory accepts ComputeApi::HIP only (Source/TunerCore.cpp:485-550 dispatches
+// OpenCL / CUDA / Vulkan in the reference).
+#pragma once
+
+#include <memory>
+#include <ostream>
+#include <string>
+#include <vector>
+
+#include <ComputeInterface.h>
+#include <Info.h>
+#include <KernelResult.h>
+#include <KttTypes.h>
+#include <Searcher.h>
+#include <StopCondition.h>
+
+#include "ComputeEngine.h"
+#include "Json.h"
+#include "Kernel.h"
+#include "KernelArgument.h"
+#include "KernelRunner.h"
+#include "TuningRunner.h"
+
+namespace ktt
+{
+
+class TunerCore
+{
+public:
+    explicit TunerCore(const PlatformIndex platform, const DeviceIndex device, const ComputeApi api, const uint32_t queueCount);
+    explicit TunerCore(const ComputeApi api, const ComputeApiInitializer& initializer, std::vector<QueueId>& assignedQueueIds);
+    ~TunerCore();
+
+    KernelArgumentManager& GetArgumentManager() { return *m_ArgumentManager; }
+    KernelManager& GetKernelManager() { return *m_KernelManager; }
+    const KernelManager& GetKernelManager() const { return *m_KernelManager; }
+    ComputeEngine& GetComputeEngine() { return *m_ComputeEngine; }
+    const ComputeEngine& GetComputeEngine() const { return *m_ComputeEngine; }
+    KernelRunner& GetKernelRunner() { return *m_KernelRunner; }
+    TuningRunner& GetTuningRunner() { return *m_TuningRunner; }
+    const TuningRunner& GetTuningRunner() const { return *m_TuningRunner; }
+
+    void RemoveKernel(const KernelId id);
+    void RemoveArgument(const ArgumentId& id);
+    std::string GetKernelSource(const KernelId id, const KernelConfiguration& configuration) const;
+    std::string GetKernelDefinitionSource(const KernelDefinitionId id, const KernelConfiguration& configuration) const;
+
+    void SaveResults(const std::vector<KernelResult>& results, const std::string& filePath, const OutputFormat format,
+        const UserData& data) const;
+    std::vector<KernelResult> LoadResults(const std::string& filePath, const OutputFormat format, UserData& data) const;
+
+    JsonValue CreateMetadata() const;
+    static void Log(const LoggingLevel level, const std::string& message);
+
+private:
+    std::unique_ptr<KernelArgumentManager> m_ArgumentManager;
+    std::unique_ptr<KernelManager> m_KernelManager;
+    std::unique_ptr<ComputeEngine> m_ComputeEngine;
+    std::unique_ptr<KernelRunner> m_KernelRunner;
+    std::unique_ptr<TuningRunner> m_TuningRunner;
+
+    void InitializeRunners();
+};
+
+/** KTT result JSON (Source/Output/JsonConverters.cpp:8-351 schema: Metadata, Results, UserData). */
+JsonValue ResultToJson(const KernelResult& result);
+KernelResult ResultFromJson(const JsonValue& value);
+std::string SerializeResults(const JsonValue& metadata, const std::vector<KernelResult>& results, const UserData& data);
+std::vector<KernelResult> DeserializeResults(const std::string& text, UserData& data, JsonValue* metadata = nullptr);
+
+} // namespace ktt

@@ -1,0 +1,198 @@
+// CoulombSum3d for gfx950: the whole 256^3-class grid in one launch.
+//
+// directCoulombSum keeps the argument list of the reference kernel
+// (Examples/CoulombSum3d/CoulombSum3d.cu:1), its z-major layout
+// (index z*G*G + y*G + x, CoulombSum3d.cu:10-11), its grid coordinates
+// (h * index, CoulombSum3d.cu:13-15) and its "+=" store (CoulombSum3d.cu:46).
+// The computation is the row/tile engine of coulomb_core.h.
+//
+// Launch (CUDA-style grid of workgroups):
+//   block = (WORK_GROUP_SIZE_X, WORK_GROUP_SIZE_Y, 1)
+//   grid  = (ceil(G / (WORK_GROUP_SIZE_X * POINTS_X)),
+//            ceil(G * G / (WORK_GROUP_SIZE_Y * ROWS_PER_THREAD)),
+//            atom blocks)
+// With more than one atom block the plain kernel adds its partial sums with
+// FP32 atomics; the compensated kernel merges them in FP64 instead.
+#include "coulomb_core.h"
+
+extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const float4* atomInfo, const float* atomInfoX,
+    const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms, float gridSpacing,
+    int gridSize, float* energyGrid)
+{
+    const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
+    const int rowsTotal = gridSize * gridSize;
+    const int rowBase = blockIdx.y * KC_ROWS;
+    const int x0 = (blockIdx.x * WORK_GROUP_SIZE_X + threadIdx.x) * POINTS_X;
+    const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
+
+    int a0, a1;
+    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, a0, a1);
+
+    float xLocal[POINTS_X];
+#pragma unroll
+    for (int i = 0; i < POINTS_X; ++i)
+    {
+        xLocal[i] = gridSpacing * static_cast<float>(x0 + i);
+    }
+
+    float part[ROWS_PER_THREAD][POINTS_X];
+    float unused[ROWS_PER_THREAD][POINTS_X];
+#pragma unroll
+    for (int k = 0; k < ROWS_PER_THREAD; ++k)
+    {
+#pragma unroll
+        for (int i = 0; i < POINTS_X; ++i)
+        {
+            part[k][i] = 0.0f;
+        }
+    }
+
+    kc::accumulateAtoms<kc::Volume, false>(src, a0, a1, gridSpacing, rowBase, rowsTotal, gridSize, 0.0, xLocal, part,
+        unused, unused);
+
+    if (gridDim.z == 1)
+    {
+        kc::storeBlock<false>(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, part);
+    }
+    else
+    {
+        kc::storeBlock<true>(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, part);
+    }
+}
+
+// Compensated form: tile-relative displacements (x relative to the
+// workgroup's first x, formed in FP64), a two-sum per point per atom tile, and
+// an FP64 merge of the atom-block partials inside this single launch.
+//
+//   partials     float2[atomBlocks * tiles * KC_THREADS * KC_POINTS], (sum, comp)
+//                per point; tiles = gridDim.x * gridDim.y.  Unused when
+//                gridDim.z == 1.
+//   tileCounters uint[tiles], zero before the first launch; every launch
+//                leaves them zero again.
+// The last workgroup of a tile to finish (counted with a device-scope atomic)
+// sums the partials of all atom blocks in FP64 and adds them to energyGrid.
+extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompensated(const float4* atomInfo,
+    const float* atomInfoX, const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms,
+    float gridSpacing, int gridSize, float* energyGrid, float2* partials, unsigned int* tileCounters)
+{
+    const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
+    const int rowsTotal = gridSize * gridSize;
+    const int rowBase = blockIdx.y * KC_ROWS;
+    const int xBase = blockIdx.x * KC_SPAN_X;
+    const int xFirst = threadIdx.x * POINTS_X;
+    const int x0 = xBase + xFirst;
+    const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
+    const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
+
+    int a0, a1;
+    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, a0, a1);
+
+    const double h = static_cast<double>(gridSpacing);
+    const double originX = h * xBase;
+    float xLocal[POINTS_X];
+#pragma unroll
+    for (int i = 0; i < POINTS_X; ++i)
+    {
+        xLocal[i] = static_cast<float>(h * (xFirst + i));
+    }
+
+    float part[ROWS_PER_THREAD][POINTS_X];
+    float sum[ROWS_PER_THREAD][POINTS_X];
+    float comp[ROWS_PER_THREAD][POINTS_X];
+#pragma unroll
+    for (int k = 0; k < ROWS_PER_THREAD; ++k)
+    {
+#pragma unroll
+        for (int i = 0; i < POINTS_X; ++i)
+        {
+            part[k][i] = 0.0f;
+            sum[k][i] = 0.0f;
+            comp[k][i] = 0.0f;
+        }
+    }
+
+    kc::accumulateAtoms<kc::Volume, true>(src, a0, a1, gridSpacing, rowBase, rowsTotal, gridSize, originX, xLocal, part,
+        sum, comp);
+
+    if (gridDim.z == 1)
+    {
+#pragma unroll
+        for (int k = 0; k < ROWS_PER_THREAD; ++k)
+        {
+            const int row = row0 + k;
+            if (row >= rowsTotal)
+            {
+                break;
+            }
+#pragma unroll
+            for (int i = 0; i < POINTS_X; ++i)
+            {
+                if (x0 + i < gridSize)
+                {
+                    float* p = energyGrid + static_cast<size_t>(row) * gridSize + x0 + i;
+                    *p = static_cast<float>(static_cast<double>(*p) + static_cast<double>(sum[k][i])
+                        + static_cast<double>(comp[k][i]));
+                }
+            }
+        }
+        return;
+    }
+
+    // Partials laid out [block][tile][point-in-thread][thread] so that the
+    // lanes of a wave store consecutive float2s.
+    const size_t tiles = static_cast<size_t>(gridDim.x) * gridDim.y;
+    const size_t tile = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+    const size_t tileStride = static_cast<size_t>(KC_THREADS) * KC_POINTS;
+    float2* mine = partials + (static_cast<size_t>(blockIdx.z) * tiles + tile) * tileStride;
+#pragma unroll
+    for (int k = 0; k < ROWS_PER_THREAD; ++k)
+    {
+#pragma unroll
+        for (int i = 0; i < POINTS_X; ++i)
+        {
+            mine[(k * POINTS_X + i) * KC_THREADS + tid] = make_float2(sum[k][i], comp[k][i]);
+        }
+    }
+
+    __shared__ unsigned int sLast;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0)
+    {
+        const unsigned int arrived = __hip_atomic_fetch_add(&tileCounters[tile], 1u, __ATOMIC_ACQ_REL,
+            __HIP_MEMORY_SCOPE_AGENT);
+        sLast = (arrived + 1 == gridDim.z) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (sLast == 0u)
+    {
+        return;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+#pragma unroll
+    for (int k = 0; k < ROWS_PER_THREAD; ++k)
+    {
+        const int row = row0 + k;
+#pragma unroll
+        for (int i = 0; i < POINTS_X; ++i)
+        {
+            double total = 0.0;
+            for (unsigned int b = 0; b < gridDim.z; ++b)
+            {
+                const float2 sc = partials[(static_cast<size_t>(b) * tiles + tile) * tileStride
+                    + (k * POINTS_X + i) * KC_THREADS + tid];
+                total += static_cast<double>(sc.x) + static_cast<double>(sc.y);
+            }
+            if (row < rowsTotal && x0 + i < gridSize)
+            {
+                float* p = energyGrid + static_cast<size_t>(row) * gridSize + x0 + i;
+                *p = static_cast<float>(static_cast<double>(*p) + total);
+            }
+        }
+    }
+    if (tid == 0)
+    {
+        __hip_atomic_store(&tileCounters[tile], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}

@@ -1,0 +1,196 @@
+"""Parity of the gfx950 Coulomb kernels with the CPU oracle, through the C ABI.
+
+Bars: the plain FP32 form within a normalised error of 1e-5 of the FP64 oracle, the
+compensated form within 1e-6 (normalised by sum_i |q_i| / r_i, as the benchmark reports it),
+and agreement with the reference-semantics kernels at the reference's own validation
+tolerance (SideBySideComparison, absolute 0.01: Examples/CoulombSum3d/CoulombSum3d.cpp:226).
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from ktt_amd import workloads
+from oracle import coulomb_oracle as oracle
+
+pytestmark = pytest.mark.gpu
+
+PLAIN_BAR = 1e-5
+COMPENSATED_BAR = 1e-6
+SIDE_BY_SIDE = 0.01
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def run_volume(capi, atoms, spacing, grid, form=0, config=None, initial=None):
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    if initial is None:
+        energy.zero()
+    else:
+        energy.upload(initial)
+    capi.coulomb3d(atoms_buffer, atoms.shape[0], spacing, grid, energy, form=form, config=config)
+    return energy.download((grid, grid, grid))
+
+
+def all_points(grid):
+    z, y, x = np.meshgrid(np.arange(grid), np.arange(grid), np.arange(grid), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+
+
+def normalised(values_grid, atoms, spacing, points):
+    reference, norms = oracle.points_f64(atoms, spacing, points)
+    sampled = values_grid[points[:, 2], points[:, 1], points[:, 0]]
+    return oracle.normalised_error(sampled, reference, norms)
+
+
+def test_device_is_gfx950(capi):
+    info = capi.device_info()
+    print("device:", info, flush=True)
+    assert info["arch"] == "gfx950"
+    assert info["compute_units"] >= 1
+
+
+CONFIGS_3D = [
+    None,
+    "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=2,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=32",
+    "WORK_GROUP_SIZE_X=16,WORK_GROUP_SIZE_Y=8,POINTS_X=8,ROWS_PER_THREAD=8,ATOM_TILE=128,INNER_UNROLL_FACTOR=4",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=1,POINTS_X=2,ROWS_PER_THREAD=16,ATOM_TILE=64,ATOM_BLOCKS=3",
+]
+
+
+@pytest.mark.parametrize("config", CONFIGS_3D)
+def test_plain_3d_matches_fp64_oracle(capi, config):
+    atoms = workloads.uniform_atoms(21, 257, box=20.0)
+    grid, spacing = 40, 0.5
+    print(f"plain 3d config={config}", flush=True)
+    result = run_volume(capi, atoms, spacing, grid, config=config)
+    worst, rms = normalised(result, atoms, spacing, all_points(grid))
+    assert worst <= PLAIN_BAR, (worst, rms)
+    cpu = oracle.coulomb3d_f32(atoms, spacing, grid)
+    assert np.max(np.abs(result - cpu)) <= SIDE_BY_SIDE
+
+
+@pytest.mark.parametrize("config", [None, "ATOM_BLOCKS=4", "POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=32,ATOM_BLOCKS=7"])
+def test_compensated_3d_matches_fp64_oracle(capi, config):
+    atoms = workloads.uniform_atoms(22, 1000, box=20.0)
+    grid, spacing = 40, 0.5
+    print(f"compensated 3d config={config}", flush=True)
+    result = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED, config=config)
+    worst, rms = normalised(result, atoms, spacing, all_points(grid))
+    assert worst <= COMPENSATED_BAR, (worst, rms)
+
+
+def test_compensated_repeated_launch_reuses_counters(capi):
+    """The per-tile arrival counters must be left at zero by every launch."""
+    atoms = workloads.uniform_atoms(23, 700, box=16.0)
+    grid, spacing = 32, 0.5
+    first = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED, config="ATOM_BLOCKS=5")
+    second = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED, config="ATOM_BLOCKS=5")
+    np.testing.assert_array_equal(first, second)
+
+
+def test_reference_kernel_matches_oracle(capi):
+    atoms = workloads.uniform_atoms(24, 300, box=16.0)
+    grid, spacing = 32, 0.5
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    energy.zero()
+    capi.coulomb3d_reference(atoms_buffer, atoms.shape[0], spacing, grid, energy)
+    result = energy.download((grid, grid, grid))
+    worst, _ = normalised(result, atoms, spacing, all_points(grid))
+    assert worst <= PLAIN_BAR
+
+
+def test_accumulates_into_grid_and_empty_atoms_leave_it(capi):
+    atoms = workloads.uniform_atoms(25, 100, box=8.0)
+    grid, spacing = 16, 0.5
+    once = run_volume(capi, atoms, spacing, grid)
+    twice = run_volume(capi, atoms, spacing, grid, initial=once)
+    np.testing.assert_allclose(twice, 2.0 * once, rtol=1e-6, atol=1e-6)
+    start = np.full((grid, grid, grid), 3.5, dtype=np.float32)
+    empty = run_volume(capi, np.zeros((0, 4), dtype=np.float32), spacing, grid, initial=start)
+    np.testing.assert_array_equal(empty, start)
+
+
+def test_known_answer_grid_of_reference_generator(capi):
+    """Examples/JSON/generate-coulomb-output.c grid (64 atoms, 256^3, h = 0.49) at its sampled points."""
+    fixture = json.loads((GOLDEN / "coulomb3d_kat.json").read_text())
+    atoms = workloads.kat_atoms(fixture["atom_count"])
+    grid = fixture["grid"]
+    result = run_volume(capi, atoms, workloads.KAT_SPACING, grid)
+    points = np.array(fixture["points"], dtype=np.int32)
+    expected = np.array(fixture["values"], dtype=np.float32)
+    got = result[points[:, 2], points[:, 1], points[:, 0]]
+    assert np.max(np.abs(got - expected)) <= SIDE_BY_SIDE
+    _, norms = oracle.points_f64(atoms, workloads.KAT_SPACING, points)
+    assert np.max(np.abs(got.astype(np.float64) - expected) / norms) <= PLAIN_BAR
+    compensated = run_volume(capi, atoms, workloads.KAT_SPACING, grid, form=capi.COMPENSATED)
+    values, norms = oracle.points_f64(atoms, workloads.KAT_SPACING, points)
+    sampled = compensated[points[:, 2], points[:, 1], points[:, 0]]
+    assert oracle.normalised_error(sampled, values, norms)[0] <= COMPENSATED_BAR
+
+
+def test_slice_2d_full_size(capi):
+    work = workloads.slice2d()
+    atoms = capi.DeviceBuffer.from_array(work.slice_atoms)
+    energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
+    energy.zero()
+    capi.coulomb2d(atoms, work.slice_atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy)
+    result = energy.download((work.grid_y, work.grid_x))
+    cpu = oracle.coulomb2d_f32(work.slice_atoms, work.spacing, work.grid_x, work.grid_y)
+    assert np.max(np.abs(result - cpu)) <= SIDE_BY_SIDE
+    points = workloads.sample_points(31, 2048, work.grid_x)[:, :2]
+    values, norms = oracle.points2d_f64(work.slice_atoms, work.spacing, points)
+    assert oracle.normalised_error(result[points[:, 1], points[:, 0]], values, norms)[0] <= PLAIN_BAR
+    reference = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
+    capi.coulomb2d_reference(atoms, work.slice_atoms.shape[0], work.spacing, work.grid_x, work.grid_y, reference)
+    assert np.max(np.abs(reference.download((work.grid_y, work.grid_x)) - result)) <= SIDE_BY_SIDE
+
+
+@pytest.mark.parametrize("config", [None, "WORK_GROUP_SIZE_X=32,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_BLOCKS=1"])
+def test_iterative_and_fused_slices_match_volume(capi, config):
+    atoms = workloads.uniform_atoms(26, 333, box=64.0)
+    grid, spacing = 128, 0.5
+    volume = run_volume(capi, atoms, spacing, grid)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    energy.zero()
+    kernel_ms, wall_ms = capi.coulomb3d_iterative(atoms, spacing, grid, energy, config=config)
+    iterative = energy.download((grid, grid, grid))
+    assert kernel_ms > 0.0 and wall_ms >= kernel_ms
+    assert np.max(np.abs(iterative - volume)) <= SIDE_BY_SIDE
+    worst, _ = normalised(iterative, atoms, spacing, workloads.sample_points(32, 1024, grid))
+    assert worst <= PLAIN_BAR
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy.zero()
+    capi.coulomb3d_slices(atoms_buffer, atoms.shape[0], spacing, grid, energy, config=config)
+    fused = energy.download((grid, grid, grid))
+    assert np.max(np.abs(fused - volume)) <= SIDE_BY_SIDE
+
+
+def test_full_size_volume_properties(capi):
+    """256^3 x 65,536 atoms: FP64 error at sampled points, linearity and sign flip."""
+    work = workloads.volume3d()
+    points = workloads.sample_points(33, 256, work.grid)
+    atoms_buffer = capi.DeviceBuffer.from_array(work.atoms)
+    energy = capi.DeviceBuffer(4 * work.grid ** 3)
+    for form, bar in ((capi.PLAIN, PLAIN_BAR), (capi.COMPENSATED, COMPENSATED_BAR)):
+        energy.zero()
+        ms = capi.coulomb3d(atoms_buffer, work.atoms.shape[0], work.spacing, work.grid, energy, form=form)
+        result = energy.download((work.grid, work.grid, work.grid))
+        worst, rms = normalised(result, work.atoms, work.spacing, points)
+        print(f"form {form}: {ms:.2f} ms, {work.interactions / ms / 1e9:.3e} interactions/s, error max {worst:.2e} rms {rms:.2e}",
+              flush=True)
+        assert worst <= bar
+    flipped = work.atoms.copy()
+    flipped[:, 3] *= -2.0
+    energy.zero()
+    capi.coulomb3d(capi.DeviceBuffer.from_array(flipped), flipped.shape[0], work.spacing, work.grid, energy)
+    scaled = energy.download((work.grid, work.grid, work.grid))
+    energy.zero()
+    capi.coulomb3d(atoms_buffer, work.atoms.shape[0], work.spacing, work.grid, energy)
+    base = energy.download((work.grid, work.grid, work.grid))
+    np.testing.assert_array_equal(scaled, np.float32(-2.0) * base)
