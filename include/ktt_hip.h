@@ -62,6 +62,9 @@ int ktt_hip_memcpy_htod(void* device_destination, const void* host_source, size_
 int ktt_hip_memcpy_dtoh(void* host_destination, const void* device_source, size_t bytes);
 /* Zero-fills bytes of device memory with a kernel (no hipMemset node, safe to capture). */
 int ktt_hip_zero(void* device_pointer, size_t bytes, void* stream);
+/* Device-to-device copy with a plain 16-byte load/store kernel (16-byte aligned pointers, bytes a
+ * multiple of 16); its profiler counters calibrate the HBM traffic figures of the benchmark. */
+int ktt_hip_copy(void* device_destination, const void* device_source, size_t bytes, void* stream);
 int ktt_hip_synchronize(void);
 int ktt_hip_stream_create(void** stream);
 int ktt_hip_stream_destroy(void* stream);
@@ -119,6 +122,46 @@ int ktt_coulomb3d_slices(const float* atoms, int n_atoms, float spacing, int gri
  * work per workgroup): grid of workgroups and workgroup shape. kind: 3 = CoulombSum3d,
  * 2 = CoulombSum2d / slice kernels (rows = grid_size). */
 int ktt_coulomb_launch_shape(int kind, int grid_size, const char* config, unsigned int grid[3], unsigned int block[3]);
+
+/* ---- 5. Batches of independent systems --------------------------------------------------- */
+
+/* A batch of independent systems, each with its own atoms, cubic grid side and spacing. The
+ * reference has no batched form (its examples run one system per tuner); this extends the
+ * CoulombSum3d kernel's semantics (Examples/CoulombSum3d/CoulombSum3d.cu:1-47) to many systems.
+ * host_atoms packs the float4 atoms of system 0, then system 1, ...; atom_counts, grid_sizes
+ * and spacings have system_count entries. At most five distinct grid sides per batch; each
+ * side is a tile-shape bucket. Config names: BUCKETb_WGX, BUCKETb_PX, BUCKETb_ROWS,
+ * BUCKETb_TILE (b = 0..4, buckets in increasing grid side) and GROUPS_PER_CU. The batch owns
+ * the device copies of the atoms and one packed output array of all grids. */
+int ktt_coulomb_batch_create(const float* host_atoms, const int* atom_counts, const int* grid_sizes, const float* spacings,
+                             int system_count, const char* config, void** batch);
+int ktt_coulomb_batch_destroy(void* batch);
+
+/* Device pointer and size of the packed grids, each system's first point in it (grid_offsets has
+ * system_count entries), the number of work items, the total grid-point x atom interactions and
+ * the tile shapes in use as NAME=VALUE,... (each output optional). */
+int ktt_coulomb_batch_info(void* batch, float** grids, size_t* grid_points, long long* grid_offsets, int* items,
+                           long long* interactions, char* shapes, size_t shapes_length);
+
+/* Clears all grids of the batch (zero-fill kernel). */
+int ktt_coulomb_batch_zero(void* batch, void* stream);
+
+/* Adds V of every system to its grid: one persistent launch in which workgroups take
+ * (system, tile) items from a device-side queue, compensated FP32. use_graph != 0 replays a HIP
+ * graph (queue reset + persistent kernel) captured on the first call. */
+int ktt_coulomb_batch_run(void* batch, int use_graph, void* stream, float* kernel_ms);
+
+/* Baseline: one ktt_coulomb3d launch per system (form KTT_COULOMB_PLAIN or _COMPENSATED), all
+ * on one stream; kernel_ms is the event time around the whole sequence. */
+int ktt_coulomb_batch_run_per_system(void* batch, int form, void* stream, float* kernel_ms);
+
+/* Tunes the tile shape of every bucket with ktt::Tuner on ComputeApi::HIP (a composite kernel
+ * and launcher over the bucket's systems; searcher as in section 4), bucket after bucket, and
+ * keeps the winners in the batch. results_file (optional) is a path prefix: the KTT JSON
+ * results of bucket b go to <prefix>_bucket<b>.json and the winners to <prefix>_winners.json.
+ * best_config (optional) receives the winning shapes as a configuration string. */
+int ktt_coulomb_batch_tune(void* batch, int searcher, int max_configurations_per_bucket, const char* results_file,
+                           char* best_config, size_t best_config_length);
 
 /* ---- 4. ktt::Tuner on ComputeApi::HIP (host buffers, full framework path) ---------------- */
 

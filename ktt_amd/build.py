@@ -45,6 +45,10 @@ KERNEL_CHECKS = [
     ("CoulombSum3dIterative.hip", "directCoulombSumSlice", "POINTS_X=2,ROWS_PER_THREAD=8"),
     ("CoulombSum3dIterativeReference.hip", "directCoulombSumReference", None),
     ("coulomb_util.hip", "kttZeroFill16", None),
+    ("coulomb_util.hip", "kttCopy16", None),
+    ("CoulombBatched.hip", "coulombBatchedPersistent", None),
+    ("CoulombBatched.hip", "coulombBatchedSystem", None),
+    ("CoulombBatched.hip", "coulombBatchedReset", None),
 ]
 
 EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative"]

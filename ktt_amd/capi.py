@@ -38,6 +38,7 @@ SIGNATURES = {
     "ktt_hip_memcpy_htod": (_c.c_int, [_vp, _vp, _c.c_size_t]),
     "ktt_hip_memcpy_dtoh": (_c.c_int, [_vp, _vp, _c.c_size_t]),
     "ktt_hip_zero": (_c.c_int, [_vp, _c.c_size_t, _vp]),
+    "ktt_hip_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "ktt_hip_synchronize": (_c.c_int, []),
     "ktt_hip_stream_create": (_c.c_int, [_c.POINTER(_vp)]),
     "ktt_hip_stream_destroy": (_c.c_int, [_vp]),
@@ -55,6 +56,14 @@ SIGNATURES = {
     "ktt_coulomb3d_tune": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_char_p,
                                       _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_double), _c.POINTER(_c.c_int),
                                       _c.POINTER(_c.c_int), _vp]),
+    "ktt_coulomb_batch_create": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_int, _c.c_char_p, _c.POINTER(_vp)]),
+    "ktt_coulomb_batch_destroy": (_c.c_int, [_vp]),
+    "ktt_coulomb_batch_info": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_size_t), _vp, _c.POINTER(_c.c_int),
+                                          _c.POINTER(_c.c_longlong), _c.c_char_p, _c.c_size_t]),
+    "ktt_coulomb_batch_zero": (_c.c_int, [_vp, _vp]),
+    "ktt_coulomb_batch_run": (_c.c_int, [_vp, _c.c_int, _vp, _fp]),
+    "ktt_coulomb_batch_run_per_system": (_c.c_int, [_vp, _c.c_int, _vp, _fp]),
+    "ktt_coulomb_batch_tune": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_char_p, _c.c_char_p, _c.c_size_t]),
 }
 
 
@@ -152,6 +161,10 @@ class DeviceBuffer:
 
     def zero(self, stream=None) -> None:
         _check(library().ktt_hip_zero(self.pointer, self.nbytes - self.nbytes % 4, stream), "ktt_hip_zero")
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None, stream=None) -> None:
+        count = min(self.nbytes, other.nbytes) if nbytes is None else nbytes
+        _check(library().ktt_hip_copy(self.pointer, other.pointer, count - count % 16, stream), "ktt_hip_copy")
 
     def close(self) -> None:
         if self.pointer is not None and self.pointer.value:
@@ -268,3 +281,82 @@ def coulomb3d_tune(host_atoms: np.ndarray, spacing: float, grid: int, searcher: 
            "ktt_coulomb3d_tune")
     return {"best_config": best.value.decode(), "best_ms": best_ms.value, "valid": valid.value, "tested": tested.value,
             "grid": output}
+
+
+class Batch:
+    """Batch of independent systems (include/ktt_hip.h section 5)."""
+
+    def __init__(self, systems, config=None):
+        """systems: iterable of objects with .atoms (N, 4) float32, .grid (side) and .spacing."""
+        systems = list(systems)
+        atoms = np.ascontiguousarray(np.concatenate([s.atoms for s in systems], axis=0) if systems else
+                                     np.zeros((0, 4), np.float32), dtype=np.float32)
+        counts = np.array([s.atoms.shape[0] for s in systems], dtype=np.int32)
+        sides = np.array([s.grid for s in systems], dtype=np.int32)
+        spacings = np.array([s.spacing for s in systems], dtype=np.float32)
+        handle = ctypes.c_void_p()
+        _check(library().ktt_coulomb_batch_create(atoms.ctypes.data_as(ctypes.c_void_p), counts.ctypes.data_as(ctypes.c_void_p),
+                                                  sides.ctypes.data_as(ctypes.c_void_p),
+                                                  spacings.ctypes.data_as(ctypes.c_void_p), len(systems), _config(config),
+                                                  ctypes.byref(handle)), "ktt_coulomb_batch_create")
+        self.handle = handle
+        self.sides = sides
+        self.count = len(systems)
+
+    def info(self) -> dict:
+        grids = ctypes.c_void_p()
+        points = ctypes.c_size_t()
+        offsets = np.zeros(self.count, dtype=np.int64)
+        items = ctypes.c_int()
+        interactions = ctypes.c_longlong()
+        shapes = ctypes.create_string_buffer(2048)
+        _check(library().ktt_coulomb_batch_info(self.handle, ctypes.byref(grids), ctypes.byref(points),
+                                                offsets.ctypes.data_as(ctypes.c_void_p), ctypes.byref(items),
+                                                ctypes.byref(interactions), shapes, len(shapes)), "ktt_coulomb_batch_info")
+        return {"grids": grids, "points": points.value, "offsets": offsets, "items": items.value,
+                "interactions": interactions.value, "shapes": shapes.value.decode()}
+
+    def zero(self, stream=None) -> None:
+        _check(library().ktt_coulomb_batch_zero(self.handle, stream), "ktt_coulomb_batch_zero")
+
+    def run(self, use_graph: bool = True, stream=None, timed: bool = True) -> float | None:
+        ms = ctypes.c_float(0.0)
+        _check(library().ktt_coulomb_batch_run(self.handle, 1 if use_graph else 0, stream, ctypes.byref(ms) if timed else None),
+               "ktt_coulomb_batch_run")
+        return ms.value if timed else None
+
+    def run_per_system(self, form: int = PLAIN, stream=None, timed: bool = True) -> float | None:
+        ms = ctypes.c_float(0.0)
+        _check(library().ktt_coulomb_batch_run_per_system(self.handle, form, stream, ctypes.byref(ms) if timed else None),
+               "ktt_coulomb_batch_run_per_system")
+        return ms.value if timed else None
+
+    def tune(self, searcher: int = SEARCHER_RANDOM, max_configurations: int = 8, results_file: str | None = None) -> str:
+        best = ctypes.create_string_buffer(2048)
+        _check(library().ktt_coulomb_batch_tune(self.handle, searcher, max_configurations,
+                                                results_file.encode() if results_file else None, best, len(best)),
+               "ktt_coulomb_batch_tune")
+        return best.value.decode()
+
+    def grids(self) -> list[np.ndarray]:
+        """Host copies of every system's grid."""
+        info = self.info()
+        total = info["points"]
+        flat = np.empty(total, dtype=np.float32)
+        _check(library().ktt_hip_memcpy_dtoh(flat.ctypes.data_as(ctypes.c_void_p), info["grids"], flat.nbytes), "download grids")
+        out = []
+        for k, side in enumerate(self.sides):
+            start = int(info["offsets"][k])
+            out.append(flat[start:start + int(side) ** 3].reshape(int(side), int(side), int(side)))
+        return out
+
+    def close(self) -> None:
+        if self.handle is not None and self.handle.value:
+            library().ktt_coulomb_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -559,6 +559,32 @@ int ktt_hip_zero(void* pointer, size_t bytes, void* stream)
     KTT_C_CATCH
 }
 
+int ktt_hip_copy(void* destination, const void* source, size_t bytes, void* stream)
+{
+    KTT_C_TRY
+    if (bytes == 0)
+    {
+        return 0;
+    }
+    if (destination == nullptr || source == nullptr)
+    {
+        return SetLastError("pointer is null");
+    }
+    if (bytes % 16 != 0 || reinterpret_cast<uintptr_t>(destination) % 16 != 0 || reinterpret_cast<uintptr_t>(source) % 16 != 0)
+    {
+        return SetLastError("ktt_hip_copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    }
+    auto kernel = CoulombRuntime::Get().GetKernel("coulomb_util.hip", "kttCopy16", {});
+    unsigned long long count = bytes / 16;
+    void* arguments[] = {&destination, &source, &count};
+    const unsigned long long groups = std::min<unsigned long long>((count + 255) / 256, 8192);
+    const unsigned int grid[3] = {static_cast<unsigned int>(groups), 1, 1};
+    const unsigned int block[3] = {256, 1, 1};
+    CoulombRuntime::Launch(*kernel, grid, block, arguments, static_cast<hipStream_t>(stream), nullptr);
+    return 0;
+    KTT_C_CATCH
+}
+
 int ktt_hip_synchronize(void)
 {
     KTT_C_TRY

@@ -13,19 +13,23 @@
 // Several atom blocks add their partial sums with FP32 atomics.
 #include "coulomb_core.h"
 
+using E = kc::DefinedEngine;
+
 extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const float4* atomInfo, const float* atomInfoX,
     const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms, float gridSpacing,
     float* energyGrid)
 {
+    __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
-    const int rowLength = gridDim.x * KC_SPAN_X;
-    const int rowsTotal = gridDim.y * KC_ROWS;
-    const int rowBase = blockIdx.y * KC_ROWS;
+    const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
+    const int rowLength = gridDim.x * E::SpanX;
+    const int rowsTotal = gridDim.y * E::Rows;
+    const int rowBase = blockIdx.y * E::Rows;
     const int x0 = (blockIdx.x * WORK_GROUP_SIZE_X + threadIdx.x) * POINTS_X;
     const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
 
     int a0, a1;
-    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, a0, a1);
+    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, ATOM_TILE, a0, a1);
 
     float xLocal[POINTS_X];
 #pragma unroll
@@ -35,25 +39,17 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const 
     }
     float part[ROWS_PER_THREAD][POINTS_X];
     float unused[ROWS_PER_THREAD][POINTS_X];
-#pragma unroll
-    for (int k = 0; k < ROWS_PER_THREAD; ++k)
-    {
-#pragma unroll
-        for (int i = 0; i < POINTS_X; ++i)
-        {
-            part[k][i] = 0.0f;
-        }
-    }
+    E::clear(part);
 
-    kc::accumulateAtoms<kc::Slice, false>(src, a0, a1, gridSpacing, rowBase, rowsTotal, rowsTotal, 0.0, xLocal, part,
+    E::accumulate<kc::Slice, false>(shared, src, a0, a1, gridSpacing, tid, rowBase, rowsTotal, rowsTotal, 0.0, xLocal, part,
         unused, unused);
 
     if (gridDim.z == 1)
     {
-        kc::storeBlock<false>(energyGrid, rowLength, 0, rowLength, rowsTotal, x0, row0, part);
+        E::store<false>(energyGrid, rowLength, 0, rowLength, rowsTotal, x0, row0, part);
     }
     else
     {
-        kc::storeBlock<true>(energyGrid, rowLength, 0, rowLength, rowsTotal, x0, row0, part);
+        E::store<true>(energyGrid, rowLength, 0, rowLength, rowsTotal, x0, row0, part);
     }
 }

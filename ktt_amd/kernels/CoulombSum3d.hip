@@ -15,18 +15,22 @@
 // FP32 atomics; the compensated kernel merges them in FP64 instead.
 #include "coulomb_core.h"
 
+using E = kc::DefinedEngine;
+
 extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const float4* atomInfo, const float* atomInfoX,
     const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms, float gridSpacing,
     int gridSize, float* energyGrid)
 {
+    __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
+    const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
     const int rowsTotal = gridSize * gridSize;
-    const int rowBase = blockIdx.y * KC_ROWS;
+    const int rowBase = blockIdx.y * E::Rows;
     const int x0 = (blockIdx.x * WORK_GROUP_SIZE_X + threadIdx.x) * POINTS_X;
     const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
 
     int a0, a1;
-    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, a0, a1);
+    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, ATOM_TILE, a0, a1);
 
     float xLocal[POINTS_X];
 #pragma unroll
@@ -37,26 +41,18 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const 
 
     float part[ROWS_PER_THREAD][POINTS_X];
     float unused[ROWS_PER_THREAD][POINTS_X];
-#pragma unroll
-    for (int k = 0; k < ROWS_PER_THREAD; ++k)
-    {
-#pragma unroll
-        for (int i = 0; i < POINTS_X; ++i)
-        {
-            part[k][i] = 0.0f;
-        }
-    }
+    E::clear(part);
 
-    kc::accumulateAtoms<kc::Volume, false>(src, a0, a1, gridSpacing, rowBase, rowsTotal, gridSize, 0.0, xLocal, part,
+    E::accumulate<kc::Volume, false>(shared, src, a0, a1, gridSpacing, tid, rowBase, rowsTotal, gridSize, 0.0, xLocal, part,
         unused, unused);
 
     if (gridDim.z == 1)
     {
-        kc::storeBlock<false>(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, part);
+        E::store<false>(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, part);
     }
     else
     {
-        kc::storeBlock<true>(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, part);
+        E::store<true>(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, part);
     }
 }
 
@@ -64,7 +60,7 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const 
 // workgroup's first x, formed in FP64), a two-sum per point per atom tile, and
 // an FP64 merge of the atom-block partials inside this single launch.
 //
-//   partials     float2[atomBlocks * tiles * KC_THREADS * KC_POINTS], (sum, comp)
+//   partials     float2[atomBlocks * tiles * threads * points], (sum, comp)
 //                per point; tiles = gridDim.x * gridDim.y.  Unused when
 //                gridDim.z == 1.
 //   tileCounters uint[tiles], zero before the first launch; every launch
@@ -75,17 +71,18 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
     const float* atomInfoX, const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms,
     float gridSpacing, int gridSize, float* energyGrid, float2* partials, unsigned int* tileCounters)
 {
+    __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
+    const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
     const int rowsTotal = gridSize * gridSize;
-    const int rowBase = blockIdx.y * KC_ROWS;
-    const int xBase = blockIdx.x * KC_SPAN_X;
+    const int rowBase = blockIdx.y * E::Rows;
+    const int xBase = blockIdx.x * E::SpanX;
     const int xFirst = threadIdx.x * POINTS_X;
     const int x0 = xBase + xFirst;
     const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
-    const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
 
     int a0, a1;
-    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, a0, a1);
+    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, ATOM_TILE, a0, a1);
 
     const double h = static_cast<double>(gridSpacing);
     const double originX = h * xBase;
@@ -99,42 +96,16 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
     float part[ROWS_PER_THREAD][POINTS_X];
     float sum[ROWS_PER_THREAD][POINTS_X];
     float comp[ROWS_PER_THREAD][POINTS_X];
-#pragma unroll
-    for (int k = 0; k < ROWS_PER_THREAD; ++k)
-    {
-#pragma unroll
-        for (int i = 0; i < POINTS_X; ++i)
-        {
-            part[k][i] = 0.0f;
-            sum[k][i] = 0.0f;
-            comp[k][i] = 0.0f;
-        }
-    }
+    E::clear(part);
+    E::clear(sum);
+    E::clear(comp);
 
-    kc::accumulateAtoms<kc::Volume, true>(src, a0, a1, gridSpacing, rowBase, rowsTotal, gridSize, originX, xLocal, part,
+    E::accumulate<kc::Volume, true>(shared, src, a0, a1, gridSpacing, tid, rowBase, rowsTotal, gridSize, originX, xLocal, part,
         sum, comp);
 
     if (gridDim.z == 1)
     {
-#pragma unroll
-        for (int k = 0; k < ROWS_PER_THREAD; ++k)
-        {
-            const int row = row0 + k;
-            if (row >= rowsTotal)
-            {
-                break;
-            }
-#pragma unroll
-            for (int i = 0; i < POINTS_X; ++i)
-            {
-                if (x0 + i < gridSize)
-                {
-                    float* p = energyGrid + static_cast<size_t>(row) * gridSize + x0 + i;
-                    *p = static_cast<float>(static_cast<double>(*p) + static_cast<double>(sum[k][i])
-                        + static_cast<double>(comp[k][i]));
-                }
-            }
-        }
+        E::storeCompensated(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, sum, comp);
         return;
     }
 
@@ -142,7 +113,7 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
     // lanes of a wave store consecutive float2s.
     const size_t tiles = static_cast<size_t>(gridDim.x) * gridDim.y;
     const size_t tile = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
-    const size_t tileStride = static_cast<size_t>(KC_THREADS) * KC_POINTS;
+    const size_t tileStride = static_cast<size_t>(E::Threads) * E::Points;
     float2* mine = partials + (static_cast<size_t>(blockIdx.z) * tiles + tile) * tileStride;
 #pragma unroll
     for (int k = 0; k < ROWS_PER_THREAD; ++k)
@@ -150,7 +121,7 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
 #pragma unroll
         for (int i = 0; i < POINTS_X; ++i)
         {
-            mine[(k * POINTS_X + i) * KC_THREADS + tid] = make_float2(sum[k][i], comp[k][i]);
+            mine[(k * POINTS_X + i) * E::Threads + tid] = make_float2(sum[k][i], comp[k][i]);
         }
     }
 
@@ -181,7 +152,7 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
             for (unsigned int b = 0; b < gridDim.z; ++b)
             {
                 const float2 sc = partials[(static_cast<size_t>(b) * tiles + tile) * tileStride
-                    + (k * POINTS_X + i) * KC_THREADS + tid];
+                    + (k * POINTS_X + i) * E::Threads + tid];
                 total += static_cast<double>(sc.x) + static_cast<double>(sc.y);
             }
             if (row < rowsTotal && x0 + i < gridSize)

@@ -15,22 +15,25 @@
 //         grid (G / (WORK_GROUP_SIZE_X * POINTS_X), G / (WORK_GROUP_SIZE_Y * ROWS_PER_THREAD), atom blocks).
 #include "coulomb_core.h"
 
+using E = kc::DefinedEngine;
+
 namespace
 {
 
 template <int MODE>
-__device__ __forceinline__ void sliceKernel(const kc::AtomSource src, const int numberOfAtoms, const float gridSpacing,
-    const int zIndex, float* energyGrid)
+__device__ __forceinline__ void sliceKernel(float* shared, const kc::AtomSource src, const int numberOfAtoms,
+    const float gridSpacing, const int zIndex, float* energyGrid)
 {
-    const int rowLength = gridDim.x * KC_SPAN_X;
-    const int rowsTotal = gridDim.y * KC_ROWS;
-    const int rowBase = blockIdx.y * KC_ROWS;
+    const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
+    const int rowLength = gridDim.x * E::SpanX;
+    const int rowsTotal = gridDim.y * E::Rows;
+    const int rowBase = blockIdx.y * E::Rows;
     const int x0 = (blockIdx.x * WORK_GROUP_SIZE_X + threadIdx.x) * POINTS_X;
     const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
     const size_t outOffset = static_cast<size_t>(rowLength) * rowsTotal * zIndex;
 
     int a0, a1;
-    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, a0, a1);
+    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, ATOM_TILE, a0, a1);
 
     float xLocal[POINTS_X];
 #pragma unroll
@@ -40,29 +43,21 @@ __device__ __forceinline__ void sliceKernel(const kc::AtomSource src, const int 
     }
     float part[ROWS_PER_THREAD][POINTS_X];
     float unused[ROWS_PER_THREAD][POINTS_X];
-#pragma unroll
-    for (int k = 0; k < ROWS_PER_THREAD; ++k)
-    {
-#pragma unroll
-        for (int i = 0; i < POINTS_X; ++i)
-        {
-            part[k][i] = 0.0f;
-        }
-    }
+    E::clear(part);
 
     // Volume mode over a single plane: row = zIndex * rowsTotal + y.
     const int planeRowBase = (MODE == kc::Volume) ? zIndex * rowsTotal + rowBase : rowBase;
     const int planeRows = (MODE == kc::Volume) ? (zIndex + 1) * rowsTotal : rowsTotal;
-    kc::accumulateAtoms<MODE, false>(src, a0, a1, gridSpacing, planeRowBase, planeRows, rowsTotal, 0.0, xLocal, part,
+    E::accumulate<MODE, false>(shared, src, a0, a1, gridSpacing, tid, planeRowBase, planeRows, rowsTotal, 0.0, xLocal, part,
         unused, unused);
 
     if (gridDim.z == 1)
     {
-        kc::storeBlock<false>(energyGrid, rowLength, outOffset, rowLength, rowsTotal, x0, row0, part);
+        E::store<false>(energyGrid, rowLength, outOffset, rowLength, rowsTotal, x0, row0, part);
     }
     else
     {
-        kc::storeBlock<true>(energyGrid, rowLength, outOffset, rowLength, rowsTotal, x0, row0, part);
+        E::store<true>(energyGrid, rowLength, outOffset, rowLength, rowsTotal, x0, row0, part);
     }
 }
 
@@ -72,14 +67,16 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const 
     const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms, float gridSpacing,
     int zIndex, float* energyGrid)
 {
+    __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
-    sliceKernel<kc::Slice>(src, numberOfAtoms, gridSpacing, zIndex, energyGrid);
+    sliceKernel<kc::Slice>(shared, src, numberOfAtoms, gridSpacing, zIndex, energyGrid);
 }
 
 extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumSlice(const float4* atomInfo,
     const float* atomInfoX, const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms,
     float gridSpacing, int zIndex, float* energyGrid)
 {
+    __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
-    sliceKernel<kc::Volume>(src, numberOfAtoms, gridSpacing, zIndex, energyGrid);
+    sliceKernel<kc::Volume>(shared, src, numberOfAtoms, gridSpacing, zIndex, energyGrid);
 }

@@ -14,6 +14,16 @@ extern "C" __global__ void kttZeroFill16(uint4* data, unsigned long long count)
     }
 }
 
+// Plain 16-byte copy; its HBM counters calibrate the profiler's FETCH_SIZE / WRITE_SIZE.
+extern "C" __global__ void kttCopy16(uint4* destination, const uint4* source, unsigned long long count)
+{
+    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    for (unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+    {
+        destination[i] = source[i];
+    }
+}
+
 extern "C" __global__ void kttZeroFill4(unsigned int* data, unsigned long long count)
 {
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
