@@ -7,13 +7,30 @@ region. Throughput is grid-point x atom interactions per second.
 
 Roofline: the kernel is bound by the FP32 vector ALU (the sum is not a contraction, so there is
 no MFMA work). Algorithmic work is 11 FLOP per interaction (3 sub, 3 mul + 2 add for r^2, rsqrt,
-q * r^-1, accumulate); the peak is MI355X's FP32 vector 157.3 TFLOP/s.
+q * r^-1, accumulate); the peak is MI355X's FP32 vector 157.3 TFLOP/s. HBM traffic per launch
+comes from the committed rocprofv3 counter summary when it is present.
 
-The CPU baseline is the oracle's FP32 restatement of the reference CPU generator
-(Examples/JSON/generate-coulomb-output.c:22-31) on all host cores, over a seeded sample of
-z-slices of the same workload; the same leg checks the GPU result against the FP64 oracle.
+Beside the headline, the same line reports (each timed with HIP events around the kernels):
+  plain        the plain FP32 form on the same workload;
+  slice2d      CoulombSum2d: one 256^2 slice at z = 31.25 A, 4,000 atoms;
+  iterative    CoulombSum3dIterative on the headline inputs: 256 slice launches with the
+               reference's host dz^2 precompute and upload per slice, and the fused device form;
+  precision    256^3 grid, 1,048,576 atoms in 64 Gaussian clusters of +-1 pairs 1 A apart,
+               plain and compensated;
+  batched      1,024 independent systems (atoms log-uniform in [200, 20000], sides in
+               {32, 48, 64, 96, 128}) in one graph-launched persistent kernel, against one plain
+               launch per system.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The CPU baseline leg times the oracle's FP32 restatement of the reference CPU generator
+(Examples/JSON/generate-coulomb-output.c:22-31) on all host cores over a seeded sample of
+z-slices of the headline workload, and checks the GPU results (headline and precision
+workloads) against the FP64 oracle at sampled grid points.
+
+--dynamic runs the dynamic-tuning workload instead (192^3 grid, 131,072 atoms, 500 steps of
+N(0, 0.05^2 I) jitter, ktt TuneIteration steps then the best configuration) and prints its own
+line; --exhaustive adds a whole-space deterministic run for the comparison.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--dynamic [--exhaustive]]
 """
 
 from __future__ import annotations
@@ -33,10 +50,13 @@ sys.path.insert(0, str(REPO))
 FLOP_PER_INTERACTION = 11
 PEAK_FP32_FLOPS = 157.3e12
 METRIC = "grid·atom interactions/s, plain FP32 vs compensated+batched, err ≤1e-6"
+TRAFFIC_SUMMARY = REPO / "profiles" / "coulomb3d_compensated_256x65536_traffic.json"
+PRECISION_POINTS = 4096
 
 
-def cpu_baseline_leg(work, gpu_grid: np.ndarray, min_seconds: float = 10.0, max_slices: int = 24) -> tuple[dict, dict]:
-    """Times the oracle on z-slices of the workload and checks the GPU grid against FP64."""
+def cpu_baseline_leg(work, gpu_grid: np.ndarray, precision: dict | None, min_seconds: float = 10.0,
+                     max_slices: int = 24) -> tuple[dict, dict]:
+    """Times the oracle on z-slices of the workload and checks GPU results against FP64."""
     from ktt_amd import workloads
     from oracle import coulomb_oracle as oracle
 
@@ -61,30 +81,38 @@ def cpu_baseline_leg(work, gpu_grid: np.ndarray, min_seconds: float = 10.0, max_
     }
     points = workloads.sample_points(33, 256, work.grid)
     values, norms = oracle.points_f64(work.atoms, work.spacing, points)
-    sampled = gpu_grid[points[:, 2], points[:, 1], points[:, 0]]
-    worst, rms = oracle.normalised_error(sampled, values, norms)
-    return baseline, {"max": worst, "rms": rms, "points": int(points.shape[0]), "against": "FP64 oracle"}
+    worst, rms = oracle.normalised_error(gpu_grid[points[:, 2], points[:, 1], points[:, 0]], values, norms)
+    errors = {"headline": {"max": worst, "rms": rms, "points": int(points.shape[0]), "against": "FP64 oracle"}}
+    if precision is not None:
+        values, norms = oracle.points_f64(precision["atoms"], precision["spacing"], precision["points"])
+        for form in ("plain", "compensated"):
+            worst, rms = oracle.normalised_error(precision["samples"][form], values, norms)
+            errors[f"precision_{form}"] = {"max": worst, "rms": rms, "points": int(precision["points"].shape[0]),
+                                           "against": "FP64 oracle"}
+    return baseline, errors
 
 
-def main() -> int:
-    parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=10)
-    parser.add_argument("--warmup", type=int, default=2)
-    parser.add_argument("--config", default=None, help="kernel configuration NAME=VALUE,... (default: tuned)")
-    parser.add_argument("--no-cpu-baseline", action="store_true")
-    args = parser.parse_args()
+def progress(message: str) -> None:
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
 
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported: the direct sum shards "
-              "by independent z-slabs with no exchange, so the multi-GPU figure is N times the one-GPU figure", file=sys.stderr)
-        return 2
 
-    from ktt_amd import capi, workloads
+def timed_loop(capi, launch, steps: int, warmup: int) -> tuple[float, float]:
+    """(kernel ms per launch from one event pair around the loop, wall ms per launch)."""
+    for _ in range(warmup):
+        launch()
+    capi.synchronize()
+    start, end = capi.Event(), capi.Event()
+    wall = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        launch()
+    end.record()
+    capi.synchronize()
+    wall = time.perf_counter() - wall
+    return start.elapsed_ms(end) / steps, 1e3 * wall / steps
 
-    capi.library()
-    info = capi.device_info()
+
+def headline_leg(capi, workloads, args):
     work = workloads.volume3d()
     n_atoms = work.atoms.shape[0]
     atoms = capi.DeviceBuffer.from_array(work.atoms)
@@ -97,36 +125,207 @@ def main() -> int:
     capi.synchronize()
 
     results = {}
+    grid_values = None
     for name, form in (("plain", capi.PLAIN), ("compensated", capi.COMPENSATED)):
         energy.zero()
-        for _ in range(args.warmup):
+
+        def launch(form=form):
             capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form, config=args.config, timed=False)
-        energy.zero()
-        capi.synchronize()
-        start, end = capi.Event(), capi.Event()
-        wall = time.perf_counter()
-        start.record()
-        for _ in range(args.steps):
-            capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form, config=args.config, timed=False)
-        end.record()
-        capi.synchronize()
-        wall = time.perf_counter() - wall
-        total_ms = start.elapsed_ms(end)
-        per_launch = total_ms / args.steps
-        results[name] = {
-            "ms_per_step": per_launch,
-            "wall_ms_per_step": 1e3 * wall / args.steps,
-            "value": work.interactions / (per_launch * 1e-3),
-        }
+
+        per_launch, wall = timed_loop(capi, launch, args.steps, args.warmup)
+        results[name] = {"ms_per_step": per_launch, "wall_ms_per_step": wall, "value": work.interactions / (per_launch * 1e-3)}
         if name == "compensated":
             # One clean pass for the error check.
             energy.zero()
-            capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form, config=args.config, timed=False)
+            launch()
             capi.synchronize()
             grid_values = energy.download((work.grid, work.grid, work.grid))
 
+    # CoulombSum3dIterative on the same inputs: host dz^2 path and fused device path.
+    energy.zero()
+    capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
+    energy.zero()
+    kernel_ms, wall_ms = capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
+    slices_ms, slices_wall = timed_loop(capi, lambda: capi.coulomb3d_slices(atoms, n_atoms, work.spacing, work.grid, energy,
+                                                                          timed=False), 1, 1)
+    iterative = {
+        "host_dz2": {"kernel_ms": kernel_ms, "wall_ms": wall_ms, "value": work.interactions / (kernel_ms * 1e-3),
+                     "value_wall": work.interactions / (wall_ms * 1e-3)},
+        "device_slices": {"ms": slices_ms, "value": work.interactions / (slices_ms * 1e-3)},
+    }
+    for buffer in (atoms, calibration, energy):
+        buffer.close()
+    return work, results, grid_values, iterative
+
+
+def slice2d_leg(capi, workloads, args) -> dict:
+    work = workloads.slice2d()
+    atoms = capi.DeviceBuffer.from_array(work.slice_atoms)
+    energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
+    energy.zero()
+    per_launch, wall = timed_loop(capi, lambda: capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x,
+                                                               work.grid_y, energy, timed=False), max(args.steps, 20), args.warmup)
+    atoms.close()
+    energy.close()
+    return {"ms": per_launch, "wall_ms": wall, "value": work.interactions / (per_launch * 1e-3),
+            "interactions": work.interactions}
+
+
+def precision_leg(capi, workloads) -> tuple[dict, dict]:
+    work = workloads.clustered()
+    n_atoms = work.atoms.shape[0]
+    atoms = capi.DeviceBuffer.from_array(work.atoms)
+    energy = capi.DeviceBuffer(4 * work.grid ** 3)
+    points = workloads.sample_points(3, PRECISION_POINTS, work.grid)
+    timing, samples = {}, {}
+    for name, form in (("plain", capi.PLAIN), ("compensated", capi.COMPENSATED)):
+        energy.zero()
+        ms = capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form)
+        grid = energy.download((work.grid, work.grid, work.grid))
+        samples[name] = grid[points[:, 2], points[:, 1], points[:, 0]].copy()
+        timing[name] = {"ms": ms, "value": work.interactions / (ms * 1e-3)}
+    atoms.close()
+    energy.close()
+    timing["atoms"] = n_atoms
+    timing["interactions"] = work.interactions
+    return timing, {"atoms": work.atoms, "spacing": work.spacing, "points": points, "samples": samples}
+
+
+def batched_leg(capi, workloads, config=None, repeats: int = 3) -> dict:
+    systems = workloads.batch()
+    batch = capi.Batch(systems, config=config)
+    info = batch.info()
+    batch.zero()
+    batch.run(use_graph=True)
+    graph = []
+    for _ in range(repeats):
+        batch.zero()
+        graph.append(batch.run(use_graph=True))
+    batch.zero()
+    batch.run_per_system(form=capi.PLAIN)
+    per_system = []
+    for _ in range(max(1, repeats - 1)):
+        batch.zero()
+        per_system.append(batch.run_per_system(form=capi.PLAIN))
+    batch.close()
+    graph_ms = float(np.median(graph))
+    per_system_ms = float(np.median(per_system))
+    return {
+        "systems": len(systems),
+        "interactions": info["interactions"],
+        "work_items": info["items"],
+        "shapes": info["shapes"],
+        "graph_ms": graph_ms,
+        "value": info["interactions"] / (graph_ms * 1e-3),
+        "per_system_plain_ms": per_system_ms,
+        "per_system_plain_value": info["interactions"] / (per_system_ms * 1e-3),
+        "speedup_vs_per_system_plain": per_system_ms / graph_ms,
+        "form": "compensated, one persistent launch from a captured HIP graph",
+    }
+
+
+def traffic_summary() -> tuple[float | None, str | None]:
+    if not TRAFFIC_SUMMARY.exists():
+        return None, None
+    summary = json.loads(TRAFFIC_SUMMARY.read_text())
+    return float(summary["hbm_bytes_per_launch"]), str(TRAFFIC_SUMMARY.relative_to(REPO))
+
+
+def dynamic_main(args) -> int:
+    from ktt_amd import capi, workloads
+
+    capi.library()
+    work = workloads.dynamic_initial()
+    steps = args.dynamic_steps
+    run = capi.coulomb3d_dynamic(work.atoms, work.spacing, work.grid, steps, workloads.DYNAMIC_SIGMA, workloads.SEED_DYNAMIC,
+                                 searcher=capi.SEARCHER_MCMC, tuning_budget=args.dynamic_budget,
+                                 results_file=args.results_prefix + "_mcmc" if args.results_prefix else None)
+    production = run["kernel_ms"][~run["tuned"]]
+    converged_ms = float(np.median(production)) if production.size else float("nan")
+    line = {
+        "metric": "grid·atom interactions/s, dynamic tuning (converged)",
+        "value": work.interactions / (converged_ms * 1e-3),
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": converged_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"dynamic: {work.grid}^3 grid, h=0.5 A, {work.atoms.shape[0]} atoms U in box, q U[-1,1]; {steps} steps of "
+                        f"N(0, {workloads.DYNAMIC_SIGMA}^2 I) jitter on the device; compensated FP32; MCMC TuneIteration for "
+                        f"{run['tested']} steps, then the best configuration",
+            "space": run["space"],
+            "tested": run["tested"],
+            "best_config": run["best_config"],
+        },
+        "tuning_wall_s": float(run["wall_ms"][run["tuned"]].sum() / 1e3),
+        "total_wall_s": float(run["wall_ms"].sum() / 1e3),
+        "roofline": {"bound": "mfma", "achieved": work.interactions / (converged_ms * 1e-3) * FLOP_PER_INTERACTION / 1e12,
+                     "peak": PEAK_FP32_FLOPS / 1e12, "unit": "TFLOP/s",
+                     "frac": work.interactions / (converged_ms * 1e-3) * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    if args.exhaustive:
+        exhaustive = capi.coulomb3d_dynamic(work.atoms, work.spacing, work.grid, run["space"] + 20, workloads.DYNAMIC_SIGMA,
+                                            workloads.SEED_DYNAMIC, searcher=capi.SEARCHER_DETERMINISTIC, tuning_budget=0,
+                                            results_file=args.results_prefix + "_exhaustive" if args.results_prefix else None)
+        best_ms = float(np.median(exhaustive["kernel_ms"][~exhaustive["tuned"]]))
+        line["exhaustive"] = {"best_config": exhaustive["best_config"], "ms": best_ms,
+                              "value": work.interactions / (best_ms * 1e-3), "tested": exhaustive["tested"],
+                              "converged_over_exhaustive": best_ms / converged_ms}
+    print(json.dumps(line))
+    return 0
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--config", default=None, help="kernel configuration NAME=VALUE,... (default: tuned)")
+    parser.add_argument("--batch-config", default=None, help="per-bucket shapes BUCKETb_WGX=...,... (default: built-in)")
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-extras", action="store_true", help="headline and CPU baseline only")
+    parser.add_argument("--dynamic", action="store_true", help="run the dynamic-tuning workload instead")
+    parser.add_argument("--exhaustive", action="store_true", help="with --dynamic: add a whole-space deterministic run")
+    parser.add_argument("--dynamic-steps", type=int, default=500)
+    parser.add_argument("--dynamic-budget", type=int, default=48)
+    parser.add_argument("--results-prefix", default=None, help="with --dynamic: KTT JSON results files prefix")
+    args = parser.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported: the direct sum shards "
+              "by independent z-slabs with no exchange, so the multi-GPU figure is N times the one-GPU figure", file=sys.stderr)
+        return 2
+    if args.dynamic:
+        return dynamic_main(args)
+
+    from ktt_amd import capi, workloads
+
+    capi.library()
+    info = capi.device_info()
+    work, results, grid_values, iterative = headline_leg(capi, workloads, args)
+    progress(f"headline: compensated {results['compensated']['ms_per_step']:.2f} ms, plain {results['plain']['ms_per_step']:.2f} ms")
+    extras, precision_samples = {}, None
+    if not args.no_extras:
+        extras["slice2d"] = slice2d_leg(capi, workloads, args)
+        extras["iterative"] = iterative
+        progress("slice2d and iterative done")
+        extras["precision"], precision_samples = precision_leg(capi, workloads)
+        progress("precision workload done")
+        extras["batched"] = batched_leg(capi, workloads, config=args.batch_config)
+        progress(f"batched: graph {extras['batched']['graph_ms']:.1f} ms, per system {extras['batched']['per_system_plain_ms']:.1f} ms")
+
     headline = results["compensated"]
     achieved = headline["value"] * FLOP_PER_INTERACTION / 1e12
+    traffic, traffic_source = traffic_summary()
     line = {
         "metric": METRIC,
         "value": headline["value"],
@@ -144,13 +343,14 @@ def main() -> int:
             "workload": "CoulombSum3d: 256^3 grid, h=0.5 A, 65,536 atoms U[0,128)^3, q U[-1,1], seeded; compensated FP32 "
                         "(two-sum per atom tile, FP64 merge); one kernel pass per step",
             "grid": work.grid,
-            "atoms": n_atoms,
+            "atoms": int(work.atoms.shape[0]),
             "interactions_per_step": work.interactions,
             "form": "compensated",
             "kernel_config": args.config or "default",
-            "device": info["name"] + " " + info["arch"],
+            "device": (info["name"] + " " + info["arch"]).strip(),
         },
         "plain": results["plain"],
+        "compensated": results["compensated"],
         "roofline": {
             "bound": "mfma",
             "pipe": "FP32 VALU (v_rsq_f32 + FMA); no MFMA, the sum is not a contraction",
@@ -159,13 +359,16 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved * 1e12 / PEAK_FP32_FLOPS,
             "flop_per_interaction": FLOP_PER_INTERACTION,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
         },
     }
+    line.update(extras)
     if not args.no_cpu_baseline:
-        baseline, error = cpu_baseline_leg(work, grid_values)
+        progress("CPU baseline and FP64 checks")
+        baseline, errors = cpu_baseline_leg(work, grid_values, precision_samples)
         line["cpu_baseline"] = baseline
-        line["error"] = error
+        line["error"] = errors
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

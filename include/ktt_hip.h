@@ -184,6 +184,25 @@ int ktt_coulomb3d_tune(const float* host_atoms, int n_atoms, float spacing, int 
                        char* best_config, size_t best_config_length, double* best_ms,
                        int* valid_count, int* tested_count, float* energy_grid);
 
+/* Dynamic tuning run of the compensated CoulombSum3d kernel with ktt::Tuner::TuneIteration
+ * (replaces the online-tuning loop a user writes around Source/Tuner.h:662-698, whose
+ * iteration is Source/TuningRunner/TuningRunner.cpp:84-128). The atoms are copied to the
+ * device once. Step s (0-based) moves every atom by a N(0, sigma^2 I) displacement drawn from
+ * (seed, s, atom index) on the device, clears the G^3 map and recomputes it. The first
+ * min(tuning_budget, space) steps each run the searcher's next configuration
+ * (tuning_budget <= 0: the whole space); every later step runs Tuner::GetBestConfiguration
+ * (Source/Tuner.h:776-782). Outputs, each optional: step_kernel_ms[steps] (Coulomb kernel
+ * time, -1 for a configuration that failed), step_wall_ms[steps] (host time of the step,
+ * compilation included), step_tuned[steps] (1 for a tuning step), best_config
+ * (NAME=VALUE,...), tested_count, space_size, final_atoms (4 * n_atoms floats) and
+ * final_grid (G^3 floats, the last step's map). results_file (optional, no extension)
+ * receives the tuning steps as KTT JSON results. */
+int ktt_coulomb3d_dynamic(const float* host_atoms, int n_atoms, float spacing, int grid_size, int steps,
+                          float sigma, unsigned long long seed, int searcher, int tuning_budget,
+                          const char* results_file, double* step_kernel_ms, double* step_wall_ms,
+                          int* step_tuned, char* best_config, size_t best_config_length,
+                          int* tested_count, int* space_size, float* final_atoms, float* final_grid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,10 @@ KERNEL_CHECKS = [
     ("CoulombSum3dIterativeReference.hip", "directCoulombSumReference", None),
     ("coulomb_util.hip", "kttZeroFill16", None),
     ("coulomb_util.hip", "kttCopy16", None),
+    ("coulomb_util.hip", "kttZeroFill4", None),
+    ("coulomb_util.hip", "kttJitterAtoms", None),
+    ("CoulombSum3d.hip", "directCoulombSumCompensated",
+     "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=8,POINTS_X=6,ROWS_PER_THREAD=4,ATOM_TILE=128"),
     ("CoulombBatched.hip", "coulombBatchedPersistent", None),
     ("CoulombBatched.hip", "coulombBatchedSystem", None),
     ("CoulombBatched.hip", "coulombBatchedReset", None),
@@ -91,6 +95,15 @@ def build_examples() -> list[Path]:
     return built
 
 
+def build_core_tests() -> Path:
+    """Host-only test program of the ktt core (tests/test_core.py runs it)."""
+    BIN_DIR.mkdir(exist_ok=True)
+    target = BIN_DIR / "ktt_core_tests"
+    _run(["g++", *CXX_FLAGS, str(CSRC / "tests" / "CoreTests.cpp"), "-o", str(target), f"-L{PACKAGE}", "-lktt_hip",
+          f"-Wl,-rpath,{PACKAGE}", *LINK_FLAGS])
+    return target
+
+
 def check_kernels() -> None:
     library = ctypes.CDLL(str(LIBRARY))
     library.ktt_hip_compile_check.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
@@ -117,6 +130,7 @@ def build(verbose: bool = True) -> None:
     if verbose:
         print("[ktt_amd.build] compiling examples", flush=True)
     build_examples()
+    build_core_tests()
     if verbose:
         print("[ktt_amd.build] compiling kernels for gfx950 through hipRTC", flush=True)
     check_kernels()

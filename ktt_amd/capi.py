@@ -56,6 +56,9 @@ SIGNATURES = {
     "ktt_coulomb3d_tune": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_char_p,
                                       _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_double), _c.POINTER(_c.c_int),
                                       _c.POINTER(_c.c_int), _vp]),
+    "ktt_coulomb3d_dynamic": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_float, _c.c_ulonglong, _c.c_int,
+                                         _c.c_int, _c.c_char_p, _vp, _vp, _vp, _c.c_char_p, _c.c_size_t,
+                                         _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _vp, _vp]),
     "ktt_coulomb_batch_create": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_int, _c.c_char_p, _c.POINTER(_vp)]),
     "ktt_coulomb_batch_destroy": (_c.c_int, [_vp]),
     "ktt_coulomb_batch_info": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_c.c_size_t), _vp, _c.POINTER(_c.c_int),
@@ -281,6 +284,34 @@ def coulomb3d_tune(host_atoms: np.ndarray, spacing: float, grid: int, searcher: 
            "ktt_coulomb3d_tune")
     return {"best_config": best.value.decode(), "best_ms": best_ms.value, "valid": valid.value, "tested": tested.value,
             "grid": output}
+
+
+def coulomb3d_dynamic(host_atoms: np.ndarray, spacing: float, grid: int, steps: int, sigma: float, seed: int,
+                      searcher: int = SEARCHER_MCMC, tuning_budget: int = 0, results_file: str | None = None,
+                      want_final: bool = False) -> dict:
+    """Dynamic tuning run (``ktt_coulomb3d_dynamic``): per-step kernel and wall times, which
+    steps tuned, the best configuration, and optionally the final atoms and map."""
+    host_atoms = np.ascontiguousarray(host_atoms, dtype=np.float32)
+    if host_atoms.ndim != 2 or host_atoms.shape[1] != 4:
+        raise ValueError("atoms must have shape (N, 4)")
+    kernel_ms = np.zeros(steps, np.float64)
+    wall_ms = np.zeros(steps, np.float64)
+    tuned = np.zeros(steps, np.int32)
+    best = ctypes.create_string_buffer(1024)
+    tested, space = ctypes.c_int(0), ctypes.c_int(0)
+    final_atoms = np.empty_like(host_atoms) if want_final else None
+    final_grid = np.empty((grid, grid, grid), np.float32) if want_final else None
+
+    def pointer(array):
+        return array.ctypes.data_as(ctypes.c_void_p) if array is not None else None
+
+    _check(library().ktt_coulomb3d_dynamic(pointer(host_atoms), host_atoms.shape[0], spacing, grid, steps, sigma, seed, searcher,
+                                           tuning_budget, results_file.encode() if results_file else None, pointer(kernel_ms),
+                                           pointer(wall_ms), pointer(tuned), best, len(best), ctypes.byref(tested),
+                                           ctypes.byref(space), pointer(final_atoms), pointer(final_grid)),
+           "ktt_coulomb3d_dynamic")
+    return {"kernel_ms": kernel_ms, "wall_ms": wall_ms, "tuned": tuned.astype(bool), "best_config": best.value.decode(),
+            "tested": tested.value, "space": space.value, "atoms": final_atoms, "grid": final_grid}
 
 
 class Batch:

@@ -1,0 +1,23 @@
+/** @file KttCoulomb.h
+  * Tuning spaces of this library's gfx950 Coulomb kernels, for ktt programs that tune them.
+  */
+#pragma once
+
+#include <cstdint>
+
+#include <Tuner.h>
+
+namespace ktt
+{
+
+/** Adds the CoulombSum3d tuning space (WORK_GROUP_SIZE_X/Y, POINTS_X, ROWS_PER_THREAD, ATOM_TILE,
+  * INNER_UNROLL_FACTOR), its thread modifiers and constraints to a kernel whose definition has
+  * the default global size (G, G*G, 1) and local size (1, 1, 1), in GlobalSizeType::CUDA. */
+void AddCoulomb3dTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefinitionId definition);
+
+/** Adds the dynamic-run space of the compensated CoulombSum3d kernel (the space above plus
+  * ATOM_BLOCKS, the atom range split over workgroups along the grid's z dimension) for a grid
+  * of gridSize^3 points; spans along x that waste more than an eighth of a row are excluded. */
+void AddCoulombDynamicTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefinitionId definition, const uint64_t gridSize);
+
+} // namespace ktt

@@ -333,12 +333,14 @@ void CoulombRuntime::Launch(const HipKernel& kernel, const unsigned int grid[3],
     }
 }
 
-// Tuned defaults (gfx950).
-static CoulombConfig Defaults3d()
+// Tuned defaults (gfx950, 256^3 grid x 65,536 atoms): the plain form keeps 2 x 16 points per
+// thread, the compensated form (three values per point) 4 x 8.
+static CoulombConfig Defaults3d(const int form = KTT_COULOMB_PLAIN)
 {
     CoulombConfig config;
-    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "4"}, {"ROWS_PER_THREAD", "16"},
-        {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
+    const bool compensated = form == KTT_COULOMB_COMPENSATED;
+    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", compensated ? "4" : "2"},
+        {"ROWS_PER_THREAD", compensated ? "8" : "16"}, {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
     return config;
 }
 
@@ -679,7 +681,7 @@ int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize
         return SetLastError("unknown form " + std::to_string(form));
     }
     auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, Defaults3d());
+    const CoulombConfig parsed = CoulombConfig::Parse(config, Defaults3d(form));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks);
     const std::vector<std::string> defines = parsed.Defines(kLaunchOnly);

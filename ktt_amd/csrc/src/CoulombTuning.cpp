@@ -9,6 +9,7 @@
 #include <vector>
 
 #include <Ktt.h>
+#include <KttCoulomb.h>
 
 #include "../../../include/ktt_hip.h"
 #include "CoulombRuntime.h"

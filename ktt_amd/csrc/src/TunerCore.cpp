@@ -285,9 +285,10 @@ static ComputationResult ComputationFromJson(const JsonValue& value)
 {
     const auto& time = TimeConfiguration::GetInstance();
     ComputationResult result(value.At("KernelFunction").AsString());
+    // Results written by KTT 2.1.0 carry no per-computation compilation overhead.
+    const double compilation = value.Contains("CompilationOverhead") ? value.At("CompilationOverhead").AsDouble() : 0.0;
     result.SetDurationData(time.ConvertToNanosecondsDouble(value.At("Duration").AsDouble()),
-        time.ConvertToNanosecondsDouble(value.At("Overhead").AsDouble()),
-        time.ConvertToNanosecondsDouble(value.At("CompilationOverhead").AsDouble()));
+        time.ConvertToNanosecondsDouble(value.At("Overhead").AsDouble()), time.ConvertToNanosecondsDouble(compilation));
     result.SetSizeData(DimensionsFromJson(value.At("GlobalSize")), DimensionsFromJson(value.At("LocalSize")));
     if (value.Contains("CompilationData"))
     {

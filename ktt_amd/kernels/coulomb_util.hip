@@ -24,6 +24,53 @@ extern "C" __global__ void kttCopy16(uint4* destination, const uint4* source, un
     }
 }
 
+namespace
+{
+
+// SplitMix64 finaliser: a counter-based hash giving independent 64-bit draws.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double unit53(const unsigned long long bits)
+{
+    return static_cast<double>(bits >> 11) * (1.0 / 9007199254740992.0);
+}
+
+} // namespace
+
+// Moves every atom by a N(0, sigma^2 I) displacement (Box-Muller on counter-based draws keyed
+// by seed, step and atom index); the charge is unchanged.
+extern "C" __global__ void kttJitterAtoms(float4* atoms, int count, unsigned long long seed, int step, float sigma)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+    {
+        return;
+    }
+    const unsigned long long key = mix64(seed ^ mix64(static_cast<unsigned long long>(step) * 0x632BE59BD9B4E019ull))
+        ^ (static_cast<unsigned long long>(i) * 0xD1B54A32D192ED03ull);
+    double normals[4];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+    {
+        const double u1 = fmax(unit53(mix64(key + 2 * k + 1)), 1.0e-300);
+        const double u2 = unit53(mix64(key + 2 * k + 2));
+        const double radius = sqrt(-2.0 * log(u1));
+        normals[2 * k] = radius * cos(6.283185307179586 * u2);
+        normals[2 * k + 1] = radius * sin(6.283185307179586 * u2);
+    }
+    float4 a = atoms[i];
+    a.x = static_cast<float>(static_cast<double>(a.x) + sigma * normals[0]);
+    a.y = static_cast<float>(static_cast<double>(a.y) + sigma * normals[1]);
+    a.z = static_cast<float>(static_cast<double>(a.z) + sigma * normals[2]);
+    atoms[i] = a;
+}
+
 extern "C" __global__ void kttZeroFill4(unsigned int* data, unsigned long long count)
 {
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;

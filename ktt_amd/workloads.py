@@ -168,15 +168,13 @@ def batch(systems: int = 1024, min_atoms: int = 200, max_atoms: int = 20000, spa
 
 
 def dynamic_initial(atom_count: int = 131072, grid: int = 192, spacing: float = 0.5, seed: int = SEED_DYNAMIC) -> Volume:
+    """Starting atoms of the dynamic run; each step's displacement is drawn on the device by
+    kttJitterAtoms from (seed, step, atom index)."""
     return Volume(uniform_atoms(seed, atom_count, grid * spacing), spacing, grid)
 
 
-def jitter(atoms: np.ndarray, step: int, sigma: float = 0.05, seed: int = SEED_DYNAMIC) -> np.ndarray:
-    """Atoms moved by a seeded N(0, sigma^2 I) displacement for the given step."""
-    moved = atoms.copy()
-    displacement = normal(seed * 7919 + step, 3 * atoms.shape[0], stream=5).reshape(-1, 3) * sigma
-    moved[:, :3] = (atoms[:, :3].astype(np.float64) + displacement).astype(np.float32)
-    return moved
+DYNAMIC_STEPS = 500
+DYNAMIC_SIGMA = 0.05
 
 
 def kat_atoms(count: int = 64) -> np.ndarray:

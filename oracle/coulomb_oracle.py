@@ -10,6 +10,9 @@ Test infrastructure only. Functions:
   brute_force_f64(atoms, spacing, shape)        numpy FP64 over a whole small grid; the
       independent cross-check of the C oracle for tiny inputs.
   normalised_error(values, reference, normalisers)  max and RMS of |V - V64| / sum_i |q_i|/r_i.
+  jitter(atoms, seed, step, sigma)              one Gaussian displacement step of the dynamic
+      workload (no reference counterpart: the workload is defined by its seed and sigma);
+      pinned by the published SplitMix64 outputs and by the normal distribution's moments.
 How each is pinned is listed in coulomb_oracle.c and exercised by tests/test_oracle.py.
 """
 
@@ -141,3 +144,36 @@ def normalised_error(values: np.ndarray, reference: np.ndarray, normalisers: np.
     """(max, RMS) of |values - reference| / normalisers."""
     error = np.abs(np.asarray(values, dtype=np.float64) - reference) / normalisers
     return float(error.max()), float(np.sqrt(np.mean(error ** 2)))
+
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    z = z + _GAMMA
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def jitter(atoms: np.ndarray, seed: int, step: int, sigma: float) -> np.ndarray:
+    """One step of the dynamic workload's displacement: every atom moves by sigma times a
+    standard normal triple from Box-Muller on counter-based SplitMix64 draws keyed by
+    (seed, step, atom index); charges are unchanged. The checker of the device kernel
+    kttJitterAtoms, computed in FP64 and rounded once to float32."""
+    atoms = np.asarray(atoms, dtype=np.float32)
+    with np.errstate(over="ignore"):
+        index = np.arange(atoms.shape[0], dtype=np.uint64)
+        step_key = _mix64(np.array([step], dtype=np.uint64) * np.uint64(0x632BE59BD9B4E019))
+        key = _mix64(np.uint64(seed) ^ step_key) ^ (index * np.uint64(0xD1B54A32D192ED03))
+        normals = []
+        for k in range(2):
+            u1 = (_mix64(key + np.uint64(2 * k + 1)) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+            u2 = (_mix64(key + np.uint64(2 * k + 2)) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+            radius = np.sqrt(-2.0 * np.log(np.maximum(u1, 1.0e-300)))
+            normals += [radius * np.cos(6.283185307179586 * u2), radius * np.sin(6.283185307179586 * u2)]
+    out = atoms.copy()
+    for axis in range(3):
+        out[:, axis] = (atoms[:, axis].astype(np.float64) + float(np.float32(sigma)) * normals[axis]).astype(np.float32)
+    return out

@@ -51,8 +51,9 @@ def test_malformed_configuration_string_is_rejected():
 
 def test_launch_shape_follows_configuration():
     grid, block = capi.launch_shape(3, 256, None)
+    # Plain-form defaults: 64 x 4 threads, 2 x 16 points per thread.
     assert block == (64, 4, 1)
-    assert grid == (1, 256 * 256 // 64, 1)
+    assert grid == (256 // 128, 256 * 256 // 64, 1)
     grid, block = capi.launch_shape(3, 100, "WORK_GROUP_SIZE_X=32,POINTS_X=2,WORK_GROUP_SIZE_Y=2,ROWS_PER_THREAD=8,ATOM_BLOCKS=3")
     assert block == (32, 2, 1)
     assert grid == (2, (100 * 100 + 15) // 16, 3)

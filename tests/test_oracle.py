@@ -160,3 +160,29 @@ def test_planted_mistakes_are_caught():
     }
     for name, wrong in mistakes.items():
         assert np.max(np.abs(wrong - values) / norms) > 1e-3, name
+
+
+def test_jitter_hash_matches_published_splitmix64_outputs():
+    # SplitMix64 seeded with 0 returns 0xE220A8397B1DCDAF, then 0x6E789E6AA1B965F4.
+    state = np.array([0, 0x9E3779B97F4A7C15], dtype=np.uint64)
+    outputs = oracle._mix64(state)
+    assert int(outputs[0]) == 0xE220A8397B1DCDAF
+    assert int(outputs[1]) == 0x6E789E6AA1B965F4
+
+
+def test_jitter_is_standard_normal_per_axis_and_independent_across_steps():
+    atoms = np.zeros((200_000, 4), np.float32)
+    atoms[:, 3] = 0.25
+    first = oracle.jitter(atoms, seed=5, step=3, sigma=1.0)
+    second = oracle.jitter(atoms, seed=5, step=4, sigma=1.0)
+    np.testing.assert_array_equal(first[:, 3], atoms[:, 3])
+    displacement = first[:, :3].astype(np.float64)
+    # Sample moments of 2e5 draws: mean within ~5 standard errors, variance and kurtosis close.
+    assert np.all(np.abs(displacement.mean(axis=0)) < 0.012)
+    assert np.all(np.abs(displacement.var(axis=0) - 1.0) < 0.02)
+    assert np.all(np.abs((displacement ** 4).mean(axis=0) - 3.0) < 0.1)
+    assert abs(np.corrcoef(displacement[:, 0], displacement[:, 1])[0, 1]) < 0.012
+    assert abs(np.corrcoef(displacement[:, 0], second[:, 0])[0, 1]) < 0.012
+    # sigma scales the displacement; the same key reproduces it.
+    np.testing.assert_allclose(oracle.jitter(atoms, 5, 3, 0.05)[:, :3], 0.05 * displacement, rtol=1e-6, atol=1e-9)
+    np.testing.assert_array_equal(oracle.jitter(atoms, 5, 3, 1.0), first)
