@@ -141,21 +141,29 @@ def headline_leg(capi, workloads, args):
             capi.synchronize()
             grid_values = energy.download((work.grid, work.grid, work.grid))
 
-    # CoulombSum3dIterative on the same inputs: host dz^2 path and fused device path.
+    for buffer in (atoms, calibration, energy):
+        buffer.close()
+    return work, results, grid_values
+
+
+def iterative_leg(capi, work) -> dict:
+    """CoulombSum3dIterative on the headline inputs: the reference's host dz^2 path (one upload
+    and launch per slice) and the fused device path (one launch per slice)."""
+    n_atoms = work.atoms.shape[0]
+    atoms = capi.DeviceBuffer.from_array(work.atoms)
+    energy = capi.DeviceBuffer(4 * work.grid ** 3)
     energy.zero()
     capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
     energy.zero()
     kernel_ms, wall_ms = capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
-    slices_ms, slices_wall = timed_loop(capi, lambda: capi.coulomb3d_slices(atoms, n_atoms, work.spacing, work.grid, energy,
-                                                                          timed=False), 1, 1)
-    iterative = {
+    slices_ms, _ = timed_loop(capi, lambda: capi.coulomb3d_slices(atoms, n_atoms, work.spacing, work.grid, energy, timed=False), 1, 1)
+    atoms.close()
+    energy.close()
+    return {
         "host_dz2": {"kernel_ms": kernel_ms, "wall_ms": wall_ms, "value": work.interactions / (kernel_ms * 1e-3),
                      "value_wall": work.interactions / (wall_ms * 1e-3)},
         "device_slices": {"ms": slices_ms, "value": work.interactions / (slices_ms * 1e-3)},
     }
-    for buffer in (atoms, calibration, energy):
-        buffer.close()
-    return work, results, grid_values, iterative
 
 
 def slice2d_leg(capi, workloads, args) -> dict:
@@ -311,12 +319,12 @@ def main() -> int:
 
     capi.library()
     info = capi.device_info()
-    work, results, grid_values, iterative = headline_leg(capi, workloads, args)
+    work, results, grid_values = headline_leg(capi, workloads, args)
     progress(f"headline: compensated {results['compensated']['ms_per_step']:.2f} ms, plain {results['plain']['ms_per_step']:.2f} ms")
     extras, precision_samples = {}, None
     if not args.no_extras:
         extras["slice2d"] = slice2d_leg(capi, workloads, args)
-        extras["iterative"] = iterative
+        extras["iterative"] = iterative_leg(capi, work)
         progress("slice2d and iterative done")
         extras["precision"], precision_samples = precision_leg(capi, workloads)
         progress("precision workload done")

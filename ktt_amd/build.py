@@ -89,7 +89,7 @@ def build_examples() -> list[Path]:
         if not source.exists():
             continue
         target = BIN_DIR / name
-        _run(["g++", *CXX_FLAGS, str(source), "-o", str(target), f"-L{PACKAGE}", "-lktt_hip",
+        _run(["g++", *CXX_FLAGS, f"-I{REPO / 'include'}", str(source), "-o", str(target), f"-L{PACKAGE}", "-lktt_hip",
               f"-Wl,-rpath,{PACKAGE}", *LINK_FLAGS])
         built.append(target)
     return built

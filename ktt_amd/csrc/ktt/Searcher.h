@@ -1,11 +1,12 @@
 /** @file Searcher.h
-  * Configuration-space searchers. The base class exposes the helpers of
-  * Source/Api/Searcher/Searcher.h:22-136; Deterministic, Random and MCMC follow
-  * DeterministicSearcher.cpp:16-20, RandomSearcher.cpp:10-19 and McmcSearcher.cpp:11-188
-  * (MCMC constants: 2 maximum differences, 10 boot iterations, 0.02 escape probability,
-  * McmcSearcher.h:56-58). The random sources are seedable here; the reference seeds
-  * them from random_device / the clock (RandomIntGenerator.inl:9-11, McmcSearcher.cpp:20),
-  * which remains the default.
+  * Configuration-space searchers. The base class offers the helpers a custom searcher uses
+  * (same interface as Source/Api/Searcher/Searcher.h:22-136). The three built-in searchers
+  * have the reference's behaviour: Deterministic walks the space in index order, Random
+  * draws unexplored configurations uniformly, and MCMC is a Metropolis-style walk over
+  * configurations that differ in at most 2 parameters, after 10 random warm-up states,
+  * escaping to a slower state with probability 0.02 (McmcSearcher.h:56-58). The random
+  * sources take an optional seed; without one they are seeded from random_device, as the
+  * reference's are (RandomIntGenerator.inl:9-11, McmcSearcher.cpp:20).
   */
 #pragma once
 
@@ -21,7 +22,7 @@
 namespace ktt
 {
 
-class ConfigurationData;
+class Exploration;
 
 class Searcher
 {
@@ -45,11 +46,11 @@ public:
     const std::set<uint64_t>& GetExploredIndices() const;
     bool IsInitialized() const;
 
-    void Initialize(const ConfigurationData& data);
+    void Initialize(const Exploration& space);
     void Reset();
 
 private:
-    const ConfigurationData* m_Data;
+    const Exploration* m_Space;
 };
 
 class DeterministicSearcher : public Searcher
@@ -62,7 +63,7 @@ public:
     KernelConfiguration GetCurrentConfiguration() const override;
 
 private:
-    uint64_t m_Index;
+    uint64_t m_Position;
 };
 
 class RandomSearcher : public Searcher
@@ -75,7 +76,7 @@ public:
     KernelConfiguration GetCurrentConfiguration() const override;
 
 private:
-    KernelConfiguration m_CurrentConfiguration;
+    KernelConfiguration m_Drawn;
 };
 
 class McmcSearcher : public Searcher
@@ -95,17 +96,15 @@ public:
     static constexpr double EscapeProbability = 0.02;
 
 private:
-    uint64_t m_Index;
-    size_t m_VisitedStatesCount;
-    size_t m_OriginState;
-    size_t m_CurrentState;
-    size_t m_Boot;
-    double m_BestTime;
-    KernelConfiguration m_Start;
-    std::map<size_t, double> m_ExecutionTimes;
-    std::mt19937_64 m_Generator;
-    std::uniform_int_distribution<size_t> m_IntDistribution;
-    std::uniform_real_distribution<double> m_ProbabilityDistribution;
+    KernelConfiguration m_StartPoint;
+    std::mt19937_64 m_Random;
+    uint64_t m_Proposal;     // configuration handed out, waiting for its result
+    uint64_t m_Chain;        // state of the chain
+    double m_ChainDuration;  // its measured duration in ns (infinite until measured)
+    size_t m_WarmupLeft;     // random states still to draw before the walk starts
+
+    uint64_t DrawIndex(const uint64_t bound);
+    uint64_t DrawUnexplored();
 };
 
 } // namespace ktt

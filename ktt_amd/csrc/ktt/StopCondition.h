@@ -35,8 +35,8 @@ public:
     std::string GetStatusString() const override;
 
 private:
-    uint64_t m_CurrentCount;
-    uint64_t m_TargetCount;
+    uint64_t m_Limit;
+    uint64_t m_Seen;
 };
 
 class ConfigurationDuration : public StopCondition
@@ -49,8 +49,8 @@ public:
     std::string GetStatusString() const override;
 
 private:
-    double m_BestDuration;
-    double m_TargetDuration;
+    double m_TargetMs;
+    double m_FastestMs;
 };
 
 class ConfigurationFraction : public StopCondition
@@ -63,11 +63,9 @@ public:
     std::string GetStatusString() const override;
 
 private:
-    uint64_t m_CurrentCount;
-    uint64_t m_TotalCount;
-    double m_TargetFraction;
-
-    double GetExploredFraction() const;
+    double m_Fraction;
+    uint64_t m_Seen;
+    uint64_t m_Space;
 };
 
 class TuningDuration : public StopCondition
@@ -80,9 +78,9 @@ public:
     std::string GetStatusString() const override;
 
 private:
-    double m_PassedTime;
-    double m_TargetTime;
-    std::chrono::steady_clock::time_point m_InitialTime;
+    double m_LimitSeconds;
+    double m_ElapsedSeconds;
+    std::chrono::steady_clock::time_point m_Begin;
 };
 
 class UnionCondition : public StopCondition
@@ -95,7 +93,7 @@ public:
     std::string GetStatusString() const override;
 
 private:
-    std::vector<std::shared_ptr<StopCondition>> m_Conditions;
+    std::vector<std::shared_ptr<StopCondition>> m_Parts;
 };
 
 } // namespace ktt

@@ -15,7 +15,6 @@
 #include <StopCondition.h>
 
 #include "Common.h"
-#include "TuningRunner.h"
 
 namespace ktt
 {
@@ -630,317 +629,6 @@ void KernelResult::TransferPowerData(const KernelResult& previousResult)
             m_Results[i].SetPowerUsage(previous[i].GetPowerUsage());
         }
     }
-}
-
-// ---------------------------------------------------------------- searchers
-
-Searcher::Searcher() : m_Data(nullptr) {}
-void Searcher::OnInitialize() {}
-void Searcher::OnReset() {}
-KernelConfiguration Searcher::GetConfiguration(const uint64_t index) const { return m_Data->GetConfigurationForIndex(index); }
-uint64_t Searcher::GetIndex(const KernelConfiguration& configuration) const { return m_Data->GetIndexForConfiguration(configuration); }
-KernelConfiguration Searcher::GetRandomConfiguration() const { return m_Data->GetRandomConfiguration(); }
-
-std::vector<KernelConfiguration> Searcher::GetNeighbourConfigurations(const KernelConfiguration& configuration,
-    const uint64_t maxDifferences, const size_t maxNeighbours) const
-{
-    return m_Data->GetNeighbourConfigurations(configuration, maxDifferences, maxNeighbours);
-}
-
-uint64_t Searcher::GetConfigurationsCount() const { return m_Data->GetTotalConfigurationsCount(); }
-
-uint64_t Searcher::GetUnexploredConfigurationsCount() const
-{
-    return m_Data->GetTotalConfigurationsCount() - m_Data->GetExploredConfigurations().size();
-}
-
-const std::set<uint64_t>& Searcher::GetExploredIndices() const { return m_Data->GetExploredConfigurations(); }
-bool Searcher::IsInitialized() const { return m_Data != nullptr; }
-
-void Searcher::Initialize(const ConfigurationData& data)
-{
-    m_Data = &data;
-    OnInitialize();
-}
-
-void Searcher::Reset()
-{
-    OnReset();
-    m_Data = nullptr;
-}
-
-DeterministicSearcher::DeterministicSearcher() : Searcher(), m_Index(0) {}
-void DeterministicSearcher::OnReset() { m_Index = 0; }
-
-bool DeterministicSearcher::CalculateNextConfiguration([[maybe_unused]] const KernelResult& previousResult)
-{
-    ++m_Index;
-    return m_Index < GetConfigurationsCount();
-}
-
-KernelConfiguration DeterministicSearcher::GetCurrentConfiguration() const { return GetConfiguration(m_Index); }
-
-RandomSearcher::RandomSearcher() : Searcher() {}
-void RandomSearcher::OnInitialize() { m_CurrentConfiguration = GetRandomConfiguration(); }
-
-bool RandomSearcher::CalculateNextConfiguration([[maybe_unused]] const KernelResult& previousResult)
-{
-    m_CurrentConfiguration = GetRandomConfiguration();
-    return true;
-}
-
-KernelConfiguration RandomSearcher::GetCurrentConfiguration() const { return m_CurrentConfiguration; }
-
-McmcSearcher::McmcSearcher(const KernelConfiguration& start) :
-    McmcSearcher(start, DefaultSearcherSeed())
-{}
-
-McmcSearcher::McmcSearcher(const KernelConfiguration& start, const uint64_t seed) :
-    Searcher(),
-    m_Index(0),
-    m_VisitedStatesCount(0),
-    m_OriginState(0),
-    m_CurrentState(0),
-    m_Boot(0),
-    m_BestTime(std::numeric_limits<double>::max()),
-    m_Start(start),
-    m_Generator(seed),
-    m_ProbabilityDistribution(0.0, 1.0)
-{}
-
-void McmcSearcher::OnInitialize()
-{
-    m_IntDistribution = std::uniform_int_distribution<size_t>(0, GetConfigurationsCount() - 1);
-    size_t initialState = 0;
-    if (m_Start.IsValid())
-    {
-        initialState = GetIndex(m_Start);
-    }
-    else
-    {
-        initialState = m_IntDistribution(m_Generator);
-        m_Boot = BootIterations;
-    }
-    m_OriginState = initialState;
-    m_CurrentState = initialState;
-    m_Index = initialState;
-}
-
-void McmcSearcher::OnReset()
-{
-    m_Index = 0;
-    m_VisitedStatesCount = 0;
-    m_OriginState = 0;
-    m_CurrentState = 0;
-    m_BestTime = std::numeric_limits<double>::max();
-    m_ExecutionTimes.clear();
-}
-
-bool McmcSearcher::CalculateNextConfiguration(const KernelResult& previousResult)
-{
-    // Walk of McmcSearcher.cpp:58-188: a boot phase of random states, then Metropolis-style moves
-    // to random unexplored neighbours (up to 2 differing parameters) of the origin, accepting a
-    // state that is not slower or, with probability 0.02, any state.
-    ++m_VisitedStatesCount;
-    const double maxTime = std::numeric_limits<double>::max();
-    m_ExecutionTimes[m_Index] = previousResult.IsValid() ? static_cast<double>(previousResult.GetTotalDuration()) : maxTime;
-
-    if (m_ExecutionTimes.find(m_CurrentState) == m_ExecutionTimes.cend())
-    {
-        m_ExecutionTimes[m_CurrentState] = maxTime;
-    }
-    if (m_ExecutionTimes.find(m_OriginState) == m_ExecutionTimes.cend())
-    {
-        m_ExecutionTimes[m_OriginState] = maxTime;
-    }
-
-    if (m_Boot > 0)
-    {
-        if (m_ExecutionTimes[m_CurrentState] <= m_ExecutionTimes[m_OriginState])
-        {
-            m_OriginState = m_CurrentState;
-            m_BestTime = m_ExecutionTimes[m_CurrentState];
-        }
-        --m_Boot;
-        if (GetExploredIndices().size() >= GetConfigurationsCount())
-        {
-            return false;
-        }
-        while (GetExploredIndices().find(m_Index) != GetExploredIndices().cend())
-        {
-            m_Index = m_IntDistribution(m_Generator);
-        }
-        m_CurrentState = m_Index;
-        return true;
-    }
-
-    if (m_ExecutionTimes[m_CurrentState] <= m_ExecutionTimes[m_OriginState]
-        || m_ProbabilityDistribution(m_Generator) < EscapeProbability)
-    {
-        m_OriginState = m_CurrentState;
-        if (m_ExecutionTimes[m_CurrentState] < m_BestTime)
-        {
-            m_BestTime = m_ExecutionTimes[m_CurrentState];
-        }
-    }
-
-    if (GetExploredIndices().size() >= GetConfigurationsCount())
-    {
-        return false;
-    }
-
-    const auto neighbourConfigurations = GetNeighbourConfigurations(GetConfiguration(m_OriginState), MaximumDifferences,
-        std::numeric_limits<size_t>::max());
-    std::vector<size_t> neighbours;
-    for (const auto& neighbour : neighbourConfigurations)
-    {
-        neighbours.push_back(GetIndex(neighbour));
-    }
-
-    if (neighbours.empty())
-    {
-        while (GetExploredIndices().find(m_OriginState) != GetExploredIndices().cend())
-        {
-            m_OriginState = m_IntDistribution(m_Generator);
-        }
-        m_Index = m_OriginState;
-        m_CurrentState = m_OriginState;
-        return true;
-    }
-
-    m_CurrentState = neighbours.at(m_IntDistribution(m_Generator) % neighbours.size());
-    m_Index = m_CurrentState;
-    return true;
-}
-
-KernelConfiguration McmcSearcher::GetCurrentConfiguration() const { return GetConfiguration(m_Index); }
-
-// ---------------------------------------------------------------- stop conditions
-
-ConfigurationCount::ConfigurationCount(const uint64_t count) :
-    m_CurrentCount(0), m_TargetCount(std::max<uint64_t>(1, count))
-{}
-
-bool ConfigurationCount::IsFulfilled() const { return m_CurrentCount >= m_TargetCount; }
-
-void ConfigurationCount::Initialize(const uint64_t configurationsCount)
-{
-    m_CurrentCount = 0;
-    m_TargetCount = std::min(m_TargetCount, configurationsCount);
-}
-
-void ConfigurationCount::Update([[maybe_unused]] const KernelResult& result) { ++m_CurrentCount; }
-
-std::string ConfigurationCount::GetStatusString() const
-{
-    return "Current count of explored configurations: " + std::to_string(m_CurrentCount) + " / " + std::to_string(m_TargetCount);
-}
-
-ConfigurationDuration::ConfigurationDuration(const double duration) :
-    m_BestDuration(std::numeric_limits<double>::max()), m_TargetDuration(std::max(0.0, duration))
-{}
-
-bool ConfigurationDuration::IsFulfilled() const { return m_BestDuration <= m_TargetDuration; }
-void ConfigurationDuration::Initialize([[maybe_unused]] const uint64_t configurationsCount) { m_BestDuration = std::numeric_limits<double>::max(); }
-
-void ConfigurationDuration::Update(const KernelResult& result)
-{
-    if (result.IsValid())
-    {
-        m_BestDuration = std::min(m_BestDuration, static_cast<double>(result.GetTotalDuration()) / 1'000'000.0);
-    }
-}
-
-std::string ConfigurationDuration::GetStatusString() const
-{
-    return "Current best known configuration duration: " + std::to_string(m_BestDuration) + "ms, target duration: "
-        + std::to_string(m_TargetDuration) + "ms";
-}
-
-ConfigurationFraction::ConfigurationFraction(const double fraction) :
-    m_CurrentCount(0), m_TotalCount(0), m_TargetFraction(std::clamp(fraction, 0.0, 1.0))
-{}
-
-bool ConfigurationFraction::IsFulfilled() const { return GetExploredFraction() >= m_TargetFraction; }
-
-void ConfigurationFraction::Initialize(const uint64_t configurationsCount)
-{
-    m_CurrentCount = 0;
-    m_TotalCount = std::max<uint64_t>(1, configurationsCount);
-}
-
-void ConfigurationFraction::Update([[maybe_unused]] const KernelResult& result) { ++m_CurrentCount; }
-
-std::string ConfigurationFraction::GetStatusString() const
-{
-    return "Current fraction of explored configurations: " + std::to_string(GetExploredFraction()) + " / "
-        + std::to_string(m_TargetFraction);
-}
-
-double ConfigurationFraction::GetExploredFraction() const
-{
-    return m_TotalCount == 0 ? 0.0 : static_cast<double>(m_CurrentCount) / static_cast<double>(m_TotalCount);
-}
-
-TuningDuration::TuningDuration(const double duration) :
-    m_PassedTime(0.0), m_TargetTime(std::max(0.0, duration))
-{}
-
-bool TuningDuration::IsFulfilled() const { return m_PassedTime > m_TargetTime; }
-
-void TuningDuration::Initialize([[maybe_unused]] const uint64_t configurationsCount)
-{
-    m_InitialTime = std::chrono::steady_clock::now();
-    m_PassedTime = 0.0;
-}
-
-void TuningDuration::Update([[maybe_unused]] const KernelResult& result)
-{
-    const auto now = std::chrono::steady_clock::now();
-    const auto milliseconds = std::chrono::duration_cast<std::chrono::milliseconds>(now - m_InitialTime).count();
-    m_PassedTime = static_cast<double>(milliseconds) / 1'000.0;
-}
-
-std::string TuningDuration::GetStatusString() const
-{
-    return "Current tuning time: " + std::to_string(m_PassedTime) + " / " + std::to_string(m_TargetTime) + " seconds";
-}
-
-UnionCondition::UnionCondition(const std::vector<std::shared_ptr<StopCondition>>& conditions) : m_Conditions(conditions) {}
-
-bool UnionCondition::IsFulfilled() const
-{
-    return std::any_of(m_Conditions.cbegin(), m_Conditions.cend(), [](const auto& condition) { return condition->IsFulfilled(); });
-}
-
-void UnionCondition::Initialize(const uint64_t configurationsCount)
-{
-    for (auto& condition : m_Conditions)
-    {
-        condition->Initialize(configurationsCount);
-    }
-}
-
-void UnionCondition::Update(const KernelResult& result)
-{
-    for (auto& condition : m_Conditions)
-    {
-        condition->Update(result);
-    }
-}
-
-std::string UnionCondition::GetStatusString() const
-{
-    std::string result;
-    for (size_t i = 0; i < m_Conditions.size(); ++i)
-    {
-        result += m_Conditions[i]->GetStatusString();
-        if (i + 1 < m_Conditions.size())
-        {
-            result += "\n";
-        }
-    }
-    return result;
 }
 
 // ---------------------------------------------------------------- logger, time, enums

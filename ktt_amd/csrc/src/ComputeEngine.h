@@ -1,8 +1,8 @@
 // The compute-engine boundary (Source/ComputeEngine/ComputeEngine.h:19-81 of the
 // reference): kernel runs, profiling, buffers, queues, information and settings.
-// KernelComputeData carries one launch (Source/ComputeEngine/KernelComputeData.cpp:15-124):
-// name, source, configuration, modified global/local sizes, arguments and the unique
-// compute id (name + templated name + define prefix) used as kernel-cache key.
+// KernelComputeData carries one launch (the reference's KernelComputeData plays this role):
+// name, source, the configuration's defines, global and local sizes after the thread
+// modifiers, bound arguments, and the key under which the engine caches the compiled kernel.
 #pragma once
 
 #include <cstdint>
@@ -20,41 +20,44 @@
 namespace ktt
 {
 
-class Kernel;
 class KernelDefinition;
 
+/** Everything the engine needs for one launch of one kernel definition. */
 class KernelComputeData
 {
 public:
-    explicit KernelComputeData(const Kernel& kernel, const KernelDefinition& definition, const KernelConfiguration& configuration,
-        const KernelDimensions& dimensions);
+    KernelComputeData(const KernelDefinition& definition, const KernelConfiguration& configuration, const DimensionVector& globalSize,
+        const DimensionVector& localSize);
 
-    void SetGlobalSize(const DimensionVector& globalSize) { m_GlobalSize = globalSize; }
-    void SetLocalSize(const DimensionVector& localSize) { m_LocalSize = localSize; }
+    void SetGlobalSize(const DimensionVector& globalSize) { m_Global = globalSize; }
+    void SetLocalSize(const DimensionVector& localSize) { m_Local = localSize; }
     void SetArguments(const std::vector<KernelArgument*>& arguments) { m_Arguments = arguments; }
+    /** Exchanges the positions of two arguments; throws if either is not bound. */
     void SwapArguments(const ArgumentId& first, const ArgumentId& second);
 
     const std::string& GetName() const { return m_Name; }
-    const std::string& GetDefaultSource() const { return m_DefaultSource; }
-    const std::string& GetConfigurationPrefix() const { return m_ConfigurationPrefix; }
-    std::string GetSource() const { return m_ConfigurationPrefix + m_DefaultSource; }
-    const std::string& GetTemplatedName() const { return m_TemplatedName; }
-    const std::string& GetSourceDirectory() const { return m_SourceDirectory; }
-    KernelComputeId GetUniqueIdentifier() const;
-    const DimensionVector& GetGlobalSize() const { return m_GlobalSize; }
-    const DimensionVector& GetLocalSize() const { return m_LocalSize; }
+    const std::string& GetTemplatedName() const { return m_Templated; }
+    /** Source without the configuration's defines (the engine passes those as -D options). */
+    const std::string& GetDefaultSource() const { return m_Source; }
+    const std::string& GetConfigurationPrefix() const { return m_Defines; }
+    std::string GetSource() const { return m_Defines + m_Source; }
+    const std::string& GetSourceDirectory() const { return m_Directory; }
+    /** Kernel-cache key: the templated name and the configuration's defines. */
+    KernelComputeId GetUniqueIdentifier() const { return m_Templated + "\n" + m_Defines; }
+    const DimensionVector& GetGlobalSize() const { return m_Global; }
+    const DimensionVector& GetLocalSize() const { return m_Local; }
     const KernelConfiguration& GetConfiguration() const { return m_Configuration; }
     const std::vector<KernelArgument*>& GetArguments() const { return m_Arguments; }
 
 private:
     std::string m_Name;
-    std::string m_DefaultSource;
-    std::string m_ConfigurationPrefix;
-    std::string m_TemplatedName;
-    std::string m_SourceDirectory;
+    std::string m_Templated;
+    std::string m_Source;
+    std::string m_Defines;
+    std::string m_Directory;
     KernelConfiguration m_Configuration;
-    DimensionVector m_GlobalSize;
-    DimensionVector m_LocalSize;
+    DimensionVector m_Global;
+    DimensionVector m_Local;
     std::vector<KernelArgument*> m_Arguments;
 };
 

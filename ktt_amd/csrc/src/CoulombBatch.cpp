@@ -697,6 +697,25 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
             winner = &result;
         }
     }
+    // The bucket's shape before tuning competes too: a short random search may not beat it.
+    const BucketShape incumbent = batch.Shapes()[static_cast<size_t>(bucket)];
+    KernelResult incumbentResult;
+    const ParameterInput incumbentInput = {{prefix + "WGX", static_cast<uint64_t>(incumbent.wgx)},
+        {prefix + "PX", static_cast<uint64_t>(incumbent.px)}, {prefix + "ROWS", static_cast<uint64_t>(incumbent.rows)},
+        {prefix + "TILE", static_cast<uint64_t>(incumbent.tile)}};
+    const auto inList = [](const int value, const std::initializer_list<int> list)
+    {
+        return std::find(list.begin(), list.end(), value) != list.end();
+    };
+    if (inList(incumbent.wgx, {16, 32, 64}) && inList(incumbent.px, {1, 2, 3, 4}) && inList(incumbent.rows, {2, 4, 8})
+        && inList(incumbent.tile, {32, 64, 128}))
+    {
+        incumbentResult = tuner.Run(kernel, tuner.CreateConfiguration(kernel, incumbentInput), {});
+        if (incumbentResult.IsValid() && (winner == nullptr || incumbentResult.GetKernelDuration() < winner->GetKernelDuration()))
+        {
+            winner = &incumbentResult;
+        }
+    }
     if (winner == nullptr)
     {
         throw KttException("no configuration of bucket " + std::to_string(bucket) + " ran successfully");

@@ -62,6 +62,52 @@ void AddCoulomb3dTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDe
     tuner.AddConstraint(kernel, {"INNER_UNROLL_FACTOR", "ATOM_TILE"}, [](const std::vector<uint64_t>& v) { return v[0] <= v[1]; });
 }
 
+void AddCoulombSliceTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefinitionId definition, const uint64_t gridX,
+    const uint64_t gridY)
+{
+    tuner.AddParameter(kernel, "WORK_GROUP_SIZE_X", std::vector<uint64_t>{16, 32, 64});
+    tuner.AddParameter(kernel, "WORK_GROUP_SIZE_Y", std::vector<uint64_t>{1, 2, 4, 8});
+    tuner.AddParameter(kernel, "POINTS_X", std::vector<uint64_t>{1, 2, 4});
+    tuner.AddParameter(kernel, "ROWS_PER_THREAD", std::vector<uint64_t>{2, 4, 8});
+    tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{32, 64, 128});
+    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1});
+    tuner.AddParameter(kernel, "ATOM_BLOCKS", std::vector<uint64_t>{1, 4, 16});
+
+    tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::X, "WORK_GROUP_SIZE_X",
+        ModifierAction::Multiply);
+    tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::Y, "WORK_GROUP_SIZE_Y",
+        ModifierAction::Multiply);
+    tuner.AddThreadModifier(kernel, {definition}, ModifierType::Global, ModifierDimension::X,
+        std::vector<std::string>{"WORK_GROUP_SIZE_X", "POINTS_X"},
+        [](const uint64_t size, const std::vector<uint64_t>& values) { return size / (values[0] * values[1]); });
+    tuner.AddThreadModifier(kernel, {definition}, ModifierType::Global, ModifierDimension::Y,
+        std::vector<std::string>{"WORK_GROUP_SIZE_Y", "ROWS_PER_THREAD"},
+        [](const uint64_t size, const std::vector<uint64_t>& values) { return size / (values[0] * values[1]); });
+    tuner.AddThreadModifier(kernel, {definition}, ModifierType::Global, ModifierDimension::Z, "ATOM_BLOCKS",
+        ModifierAction::Multiply);
+
+    tuner.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y"},
+        [](const std::vector<uint64_t>& v) { return v[0] * v[1] >= 64 && v[0] * v[1] <= 1024; });
+    // The slice kernels take the row length and row count from the launch: both must tile exactly.
+    tuner.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "POINTS_X"},
+        [gridX](const std::vector<uint64_t>& v) { return gridX % (v[0] * v[1]) == 0; });
+    tuner.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y", "ROWS_PER_THREAD"},
+        [gridY](const std::vector<uint64_t>& v)
+        {
+            const uint64_t threads = v[0] * v[1];
+            const uint64_t rows = v[1] * v[2];
+            return gridY % rows == 0 && (threads % rows == 0 || rows % threads == 0);
+        });
+    tuner.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y", "ROWS_PER_THREAD", "ATOM_TILE"},
+        [](const std::vector<uint64_t>& v)
+        {
+            const uint64_t threads = v[0] * v[1];
+            const uint64_t rows = v[1] * v[2];
+            const uint64_t stride = threads > rows ? threads / rows : 1;
+            return v[3] % stride == 0 && v[3] * (rows + 4) * 4 <= 64 * 1024;
+        });
+}
+
 } // namespace ktt
 
 using namespace ktt;

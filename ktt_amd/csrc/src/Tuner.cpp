@@ -206,7 +206,7 @@ void Tuner::RemoveArgument(const ArgumentId& id)
 void Tuner::SetReadOnlyArgumentCache(const bool flag)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetReadOnlyArgumentCache(flag);
+    m_Tuner->GetExecutor().KeepReadOnlyBuffers(flag);
     KTT_CATCH
 }
 
@@ -220,56 +220,56 @@ KernelResult Tuner::Run(const KernelId id, const KernelConfiguration& configurat
 {
     KTT_TRY
     const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetKernelRunner().RunKernel(kernel, configuration, dimensions, KernelRunMode::Running, output);
+    return m_Tuner->GetExecutor().Execute(kernel, configuration, dimensions, KernelRunMode::Running, output);
     KTT_CATCH_RETURN(KernelResult())
 }
 
 void Tuner::SetProfiling(const bool flag)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetProfiling(flag);
+    m_Tuner->GetExecutor().SetProfiling(flag);
     KTT_CATCH
 }
 
 bool Tuner::GetProfiling()
 {
     KTT_TRY
-    return m_Tuner->GetKernelRunner().IsProfilingActive();
+    return m_Tuner->GetExecutor().IsProfiling();
     KTT_CATCH_RETURN(false)
 }
 
 void Tuner::SetValidationMethod(const ValidationMethod method, const double toleranceThreshold)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetValidationMethod(method, toleranceThreshold);
+    m_Tuner->GetExecutor().Checker().SetMethod(method, toleranceThreshold);
     KTT_CATCH
 }
 
 void Tuner::SetValidationMode(const ValidationMode mode)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetValidationMode(mode);
+    m_Tuner->GetExecutor().Checker().SetModes(mode);
     KTT_CATCH
 }
 
 void Tuner::SetValidationRange(const ArgumentId& id, const size_t range)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetValidationRange(id, range);
+    m_Tuner->GetExecutor().Checker().SetRange(m_Tuner->GetArgumentManager().GetArgument(id), range);
     KTT_CATCH
 }
 
 void Tuner::SetValueComparator(const ArgumentId& id, ValueComparator comparator)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetValueComparator(id, comparator);
+    m_Tuner->GetExecutor().Checker().SetComparator(m_Tuner->GetArgumentManager().GetArgument(id), comparator);
     KTT_CATCH
 }
 
 void Tuner::SetReferenceComputation(const ArgumentId& id, ReferenceComputation computation)
 {
     KTT_TRY
-    m_Tuner->GetKernelRunner().SetReferenceComputation(id, computation);
+    m_Tuner->GetExecutor().Checker().SetComputation(m_Tuner->GetArgumentManager().GetArgument(id), computation);
     KTT_CATCH
 }
 
@@ -278,7 +278,7 @@ void Tuner::SetReferenceKernel(const ArgumentId& id, const KernelId referenceId,
 {
     KTT_TRY
     const auto& kernel = m_Tuner->GetKernelManager().GetKernel(referenceId);
-    m_Tuner->GetKernelRunner().SetReferenceKernel(id, kernel, configuration, dimensions);
+    m_Tuner->GetExecutor().Checker().SetKernel(m_Tuner->GetArgumentManager().GetArgument(id), kernel, configuration, dimensions);
     KTT_CATCH
 }
 
@@ -286,7 +286,7 @@ void Tuner::SetReferenceArgument(const ArgumentId& id, const ArgumentId& referen
 {
     KTT_TRY
     const auto& argument = m_Tuner->GetArgumentManager().GetArgument(referenceId);
-    m_Tuner->GetKernelRunner().SetReferenceArgument(id, argument);
+    m_Tuner->GetExecutor().Checker().SetArgument(m_Tuner->GetArgumentManager().GetArgument(id), argument);
     KTT_CATCH
 }
 
@@ -299,7 +299,7 @@ std::vector<KernelResult> Tuner::Tune(const KernelId id, const KernelDimensions&
 {
     KTT_TRY
     const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetTuningRunner().Tune(kernel, dimensions, std::move(stopCondition));
+    return m_Tuner->GetTuningLoop().Tune(kernel, dimensions, std::move(stopCondition));
     KTT_CATCH_RETURN(std::vector<KernelResult>{})
 }
 
@@ -313,7 +313,7 @@ KernelResult Tuner::TuneIteration(const KernelId id, const KernelDimensions& dim
 {
     KTT_TRY
     const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetTuningRunner().TuneIteration(kernel, dimensions, KernelRunMode::OnlineTuning, output, recomputeReference);
+    return m_Tuner->GetTuningLoop().Step(kernel, dimensions, KernelRunMode::OnlineTuning, output, recomputeReference);
     KTT_CATCH_RETURN(KernelResult())
 }
 
@@ -322,7 +322,7 @@ std::vector<KernelResult> Tuner::SimulateTuning(const KernelId id, const std::ve
 {
     KTT_TRY
     const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetTuningRunner().SimulateTuning(kernel, results, std::move(stopCondition));
+    return m_Tuner->GetTuningLoop().Replay(kernel, results, std::move(stopCondition));
     KTT_CATCH_RETURN(std::vector<KernelResult>{})
 }
 
@@ -337,35 +337,35 @@ void Tuner::SetSearcher(const KernelId id, std::unique_ptr<Searcher> searcher)
     {
         throw KttException("Searcher must be a valid object");
     }
-    m_Tuner->GetTuningRunner().SetSearcher(id, std::move(searcher));
+    m_Tuner->GetTuningLoop().SetSearcher(id, std::move(searcher));
     KTT_CATCH
 }
 
 void Tuner::InitializeConfigurationData(const KernelId id)
 {
     KTT_TRY
-    m_Tuner->GetTuningRunner().InitializeConfigurationData(m_Tuner->GetKernelManager().GetKernel(id));
+    m_Tuner->GetTuningLoop().Prepare(m_Tuner->GetKernelManager().GetKernel(id));
     KTT_CATCH
 }
 
 void Tuner::ClearConfigurationData(const KernelId id)
 {
     KTT_TRY
-    m_Tuner->GetTuningRunner().ClearConfigurationData(id);
+    m_Tuner->GetTuningLoop().Reset(id);
     KTT_CATCH
 }
 
 uint64_t Tuner::GetConfigurationsCount(const KernelId id) const
 {
     KTT_TRY
-    return m_Tuner->GetTuningRunner().GetConfigurationsCount(id);
+    return m_Tuner->GetTuningLoop().SpaceSize(id);
     KTT_CATCH_RETURN(0)
 }
 
 KernelConfiguration Tuner::GetBestConfiguration(const KernelId id) const
 {
     KTT_TRY
-    return m_Tuner->GetTuningRunner().GetBestConfiguration(id);
+    return m_Tuner->GetTuningLoop().Best(id);
     KTT_CATCH_RETURN(KernelConfiguration())
 }
 

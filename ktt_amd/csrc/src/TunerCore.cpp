@@ -29,7 +29,7 @@ TunerCore::TunerCore(const PlatformIndex platform, const DeviceIndex device, con
     m_KernelManager(std::make_unique<KernelManager>(*m_ArgumentManager))
 {
     m_ComputeEngine = CreateEngine(platform, device, api, queueCount);
-    InitializeRunners();
+    CreateDrivers();
 }
 
 TunerCore::TunerCore(const ComputeApi api, const ComputeApiInitializer& initializer, std::vector<QueueId>& assignedQueueIds) :
@@ -41,28 +41,28 @@ TunerCore::TunerCore(const ComputeApi api, const ComputeApiInitializer& initiali
         throw KttException("Compute API " + ComputeApiToString(api) + " is not available in this build; use ComputeApi::HIP");
     }
     m_ComputeEngine = std::make_unique<HipEngine>(initializer, assignedQueueIds);
-    InitializeRunners();
+    CreateDrivers();
 }
 
 TunerCore::~TunerCore()
 {
-    // Runners reference the engine; destroy them first.
-    m_TuningRunner.reset();
-    m_KernelRunner.reset();
+    // The loop and the executor hold references to the engine.
+    m_TuningLoop.reset();
+    m_Executor.reset();
     m_ComputeEngine.reset();
 }
 
-void TunerCore::InitializeRunners()
+void TunerCore::CreateDrivers()
 {
-    m_KernelRunner = std::make_unique<KernelRunner>(*m_ComputeEngine, *m_ArgumentManager);
-    m_TuningRunner = std::make_unique<TuningRunner>(*m_KernelRunner);
+    m_Executor = std::make_unique<Executor>(*m_ComputeEngine, *m_ArgumentManager);
+    m_TuningLoop = std::make_unique<TuningLoop>(*m_Executor);
 }
 
 void TunerCore::RemoveKernel(const KernelId id)
 {
     m_KernelManager->RemoveKernel(id);
-    m_TuningRunner->ClearConfigurationData(id, true);
-    m_KernelRunner->RemoveKernelData(id);
+    m_TuningLoop->Reset(id, true);
+    m_Executor->ForgetKernel(id);
 }
 
 void TunerCore::RemoveArgument(const ArgumentId& id)
@@ -72,7 +72,7 @@ void TunerCore::RemoveArgument(const ArgumentId& id)
         throw KttException("Argument with id " + id + " cannot be removed because it is still referenced by at least one kernel definition");
     }
     m_ComputeEngine->ClearBuffer(id);
-    m_KernelRunner->RemoveValidationData(id);
+    m_Executor->Checker().Forget(id);
     m_ArgumentManager->RemoveArgument(id);
 }
 

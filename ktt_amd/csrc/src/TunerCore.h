@@ -1,6 +1,6 @@
-// Owner of the managers, the compute engine and the runners (Source/TunerCore.cpp:21-35).
-// The engine factory accepts ComputeApi::HIP only (Source/TunerCore.cpp:485-550 dispatches
-// OpenCL / CUDA / Vulkan in the reference).
+// Owner of the argument and kernel registries, the compute engine, the executor and the tuning
+// loop. The engine factory accepts ComputeApi::HIP only (the reference dispatches OpenCL, CUDA
+// and Vulkan in Source/TunerCore.cpp:485-550).
 #pragma once
 
 #include <memory>
@@ -19,8 +19,8 @@
 #include "Json.h"
 #include "Kernel.h"
 #include "KernelArgument.h"
-#include "KernelRunner.h"
-#include "TuningRunner.h"
+#include "Execution.h"
+#include "Exploration.h"
 
 namespace ktt
 {
@@ -37,9 +37,9 @@ public:
     const KernelManager& GetKernelManager() const { return *m_KernelManager; }
     ComputeEngine& GetComputeEngine() { return *m_ComputeEngine; }
     const ComputeEngine& GetComputeEngine() const { return *m_ComputeEngine; }
-    KernelRunner& GetKernelRunner() { return *m_KernelRunner; }
-    TuningRunner& GetTuningRunner() { return *m_TuningRunner; }
-    const TuningRunner& GetTuningRunner() const { return *m_TuningRunner; }
+    Executor& GetExecutor() { return *m_Executor; }
+    TuningLoop& GetTuningLoop() { return *m_TuningLoop; }
+    const TuningLoop& GetTuningLoop() const { return *m_TuningLoop; }
 
     void RemoveKernel(const KernelId id);
     void RemoveArgument(const ArgumentId& id);
@@ -57,10 +57,10 @@ private:
     std::unique_ptr<KernelArgumentManager> m_ArgumentManager;
     std::unique_ptr<KernelManager> m_KernelManager;
     std::unique_ptr<ComputeEngine> m_ComputeEngine;
-    std::unique_ptr<KernelRunner> m_KernelRunner;
-    std::unique_ptr<TuningRunner> m_TuningRunner;
+    std::unique_ptr<Executor> m_Executor;
+    std::unique_ptr<TuningLoop> m_TuningLoop;
 
-    void InitializeRunners();
+    void CreateDrivers();
 };
 
 /** KTT result JSON (Source/Output/JsonConverters.cpp:8-351 schema: Metadata, Results, UserData). */

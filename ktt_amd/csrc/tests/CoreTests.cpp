@@ -23,7 +23,7 @@
 #include "Kernel.h"
 #include "KernelArgument.h"
 #include "TunerCore.h"
-#include "TuningRunner.h"
+#include "Exploration.h"
 
 using namespace ktt;
 
@@ -109,47 +109,29 @@ std::vector<KernelResult> LoadFixture(const std::string& path)
     return results;
 }
 
-/** Simulated tuning as in TuningRunner::SimulateTuning: every configuration the searcher asks
- *  for is answered with the stored result. Returns the explored configurations in order. */
-std::vector<std::string> Simulate(ConfigurationManager& configurations, const Kernel& kernel, const std::vector<KernelResult>& results,
-    StopCondition* stop, std::vector<double>* bestSoFar = nullptr)
+/** Simulated tuning over saved results (ReplayResults, the engine-free core of
+ *  Tuner::SimulateTuning) with a fresh exploration of the kernel's space. Returns the explored
+ *  configurations in order; `best` receives the exploration's best configuration. */
+std::vector<std::string> Simulate(const Kernel& kernel, Searcher& searcher, const std::vector<KernelResult>& results,
+    StopCondition* stop, std::vector<double>* bestSoFar = nullptr, KernelConfiguration* best = nullptr)
 {
-    std::map<std::string, const KernelResult*> byConfiguration;
-    for (const auto& result : results)
-    {
-        byConfiguration[Canonical(result.GetConfiguration())] = &result;
-    }
-    configurations.InitializeData(kernel);
-    if (stop != nullptr)
-    {
-        stop->Initialize(configurations.GetTotalConfigurationsCount(kernel.GetId()));
-    }
+    Exploration exploration(kernel, searcher);
+    const std::vector<KernelResult> replayed = ReplayResults(exploration, kernel, results, stop);
     std::vector<std::string> order;
-    double best = 1.0e300;
-    while (!configurations.IsDataProcessed(kernel.GetId()))
+    double fastest = 1.0e300;
+    for (const auto& result : replayed)
     {
-        if (stop != nullptr && stop->IsFulfilled())
-        {
-            break;
-        }
-        const std::string key = Canonical(configurations.GetCurrentConfiguration(kernel.GetId()));
-        const auto found = byConfiguration.find(key);
-        CHECK(found != byConfiguration.end());
-        if (found == byConfiguration.end())
-        {
-            break;
-        }
-        order.push_back(key);
-        best = std::min(best, static_cast<double>(found->second->GetTotalDuration()));
+        CHECK(result.IsValid());
+        order.push_back(Canonical(result.GetConfiguration()));
+        fastest = std::min(fastest, static_cast<double>(result.GetTotalDuration()));
         if (bestSoFar != nullptr)
         {
-            bestSoFar->push_back(best);
+            bestSoFar->push_back(fastest);
         }
-        configurations.CalculateNextConfiguration(kernel.GetId(), *found->second);
-        if (stop != nullptr)
-        {
-            stop->Update(*found->second);
-        }
+    }
+    if (best != nullptr)
+    {
+        *best = exploration.Best();
     }
     return order;
 }
@@ -168,15 +150,19 @@ void TestSpaceSizes()
     KernelManager manager(arguments);
     const KernelId cuda = AddReferenceCoulomb3dSpace(manager, false);
     const KernelId openCl = AddReferenceCoulomb3dSpace(manager, true);
-    ConfigurationManager configurations;
-    configurations.SetSearcher(cuda, std::make_unique<DeterministicSearcher>());
-    configurations.SetSearcher(openCl, std::make_unique<DeterministicSearcher>());
-    configurations.InitializeData(manager.GetKernel(cuda));
-    configurations.InitializeData(manager.GetKernel(openCl));
+    DeterministicSearcher first;
+    DeterministicSearcher second;
+    const Exploration cudaSpace(manager.GetKernel(cuda), first);
+    const Exploration openClSpace(manager.GetKernel(openCl), second);
     // 5 work-group shapes x 21 (unroll < z-iterations) pairs x 2 layouts; the OpenCL variant
     // adds constant memory (x2) and 6 (layout, vector) pairs instead of 2.
-    CHECK(configurations.GetTotalConfigurationsCount(cuda) == 210);
-    CHECK(configurations.GetTotalConfigurationsCount(openCl) == 1260);
+    CHECK(cudaSpace.Size() == 210);
+    CHECK(openClSpace.Size() == 1260);
+    // Index and configuration map onto each other.
+    for (uint64_t index = 0; index < cudaSpace.Size(); index += 7)
+    {
+        CHECK(cudaSpace.IndexOf(cudaSpace.ConfigurationAt(index)) == index);
+    }
 }
 
 void TestFixtureSpaceAndBest(const std::vector<KernelResult>& results)
@@ -202,25 +188,24 @@ void TestFixtureSpaceAndBest(const std::vector<KernelResult>& results)
     }
     CHECK(dumped.size() == 210);
 
-    ConfigurationManager configurations;
-    configurations.SetSearcher(kernel, std::make_unique<DeterministicSearcher>());
-    const auto order = Simulate(configurations, manager.GetKernel(kernel), results, nullptr);
+    DeterministicSearcher searcher;
+    KernelConfiguration best;
+    const auto order = Simulate(manager.GetKernel(kernel), searcher, results, nullptr, nullptr, &best);
     CHECK(order.size() == 210);
     CHECK(std::set<std::string>(order.begin(), order.end()) == dumped);
-    const KernelConfiguration best = configurations.GetBestConfiguration(kernel);
     CHECK(Canonical(best) == "INNER_UNROLL_FACTOR=0;USE_CONSTANT_MEMORY=0;USE_SOA=0;VECTOR_SIZE=1;WORK_GROUP_SIZE_X=32;"
         "WORK_GROUP_SIZE_Y=2;WORK_GROUP_SIZE_Z=1;Z_ITERATIONS=16;");
-    const KernelResult* bestResult = nullptr;
+    const KernelResult* fastest = nullptr;
     for (const auto& result : results)
     {
-        if (bestResult == nullptr || result.GetTotalDuration() < bestResult->GetTotalDuration())
+        if (fastest == nullptr || result.GetTotalDuration() < fastest->GetTotalDuration())
         {
-            bestResult = &result;
+            fastest = &result;
         }
     }
     // 769.696 us, the published best (coulomb_2080_full_search_space.json).
-    CHECK(bestResult->GetTotalDuration() == 769696);
-    CHECK(Canonical(bestResult->GetConfiguration()) == Canonical(best));
+    CHECK(fastest->GetTotalDuration() == 769696);
+    CHECK(Canonical(fastest->GetConfiguration()) == Canonical(best));
 }
 
 void TestRandomAndMcmc(const std::vector<KernelResult>& results)
@@ -234,22 +219,23 @@ void TestRandomAndMcmc(const std::vector<KernelResult>& results)
     std::vector<std::vector<std::string>> orders;
     for (int run = 0; run < 2; ++run)
     {
-        ConfigurationManager configurations;
-        configurations.SetSearcher(kernel, std::make_unique<RandomSearcher>());
-        orders.push_back(Simulate(configurations, definition, results, nullptr));
+        RandomSearcher searcher;
+        KernelConfiguration best;
+        orders.push_back(Simulate(definition, searcher, results, nullptr, nullptr, &best));
         CHECK(orders.back().size() == 210);
         CHECK(std::set<std::string>(orders.back().begin(), orders.back().end()).size() == 210);
-        CHECK(Canonical(configurations.GetBestConfiguration(kernel)).find("Z_ITERATIONS=16;") != std::string::npos);
+        CHECK(Canonical(best).find("Z_ITERATIONS=16;") != std::string::npos);
     }
     CHECK(orders[0] == orders[1]);
 
-    // MCMC: no revisits; the best-so-far duration never increases; coverage when unbounded.
-    for (const uint64_t seed : {3ull, 11ull})
+    // MCMC: no revisits; the best-so-far duration never increases; coverage when unbounded;
+    // the same seed gives the same walk.
+    std::vector<std::vector<std::string>> walks;
+    for (const uint64_t seed : {3ull, 11ull, 3ull})
     {
-        ConfigurationManager configurations;
-        configurations.SetSearcher(kernel, std::make_unique<McmcSearcher>(KernelConfiguration(), seed));
+        McmcSearcher searcher(KernelConfiguration(), seed);
         std::vector<double> best;
-        const auto order = Simulate(configurations, definition, results, nullptr, &best);
+        const auto order = Simulate(definition, searcher, results, nullptr, &best);
         CHECK(order.size() == 210);
         CHECK(std::set<std::string>(order.begin(), order.end()).size() == order.size());
         for (size_t i = 1; i < best.size(); ++i)
@@ -257,44 +243,67 @@ void TestRandomAndMcmc(const std::vector<KernelResult>& results)
             CHECK(best[i] <= best[i - 1]);
         }
         CHECK(best.back() == 769696.0);
+        walks.push_back(order);
     }
+    CHECK(walks[0] == walks[2]);
+    CHECK(walks[0] != walks[1]);
     CHECK(McmcSearcher::MaximumDifferences == 2 && McmcSearcher::BootIterations == 10 && McmcSearcher::EscapeProbability == 0.02);
+
+    // After the 10 warm-up states the walk moves to neighbours of its state: with a known start
+    // and no warm-up, the second proposal differs from the start in at most 2 parameters.
+    {
+        DeterministicSearcher probe;
+        const Exploration space(definition, probe);
+        const KernelConfiguration start = space.ConfigurationAt(100);
+        McmcSearcher searcher(start, 5);
+        Exploration walk(definition, searcher);
+        CHECK(Canonical(walk.Proposal()) == Canonical(start));
+        KernelResult first(definition.GetName(), start);
+        walk.Record(first);
+        const KernelConfiguration next = walk.Proposal();
+        size_t differences = 0;
+        for (const auto& pair : next.GetPairs())
+        {
+            for (const auto& other : start.GetPairs())
+            {
+                differences += pair.GetName() == other.GetName() && !pair.HasSameValue(other) ? 1 : 0;
+            }
+        }
+        CHECK(differences >= 1 && differences <= 2);
+    }
 }
 
 void TestStopConditions(const std::vector<KernelResult>& results)
 {
     KernelArgumentManager arguments;
     KernelManager manager(arguments);
-    const KernelId kernel = AddReferenceCoulomb3dSpace(manager, false);
+    const Kernel& kernel = manager.GetKernel(AddReferenceCoulomb3dSpace(manager, false));
     {
-        ConfigurationManager configurations;
-        configurations.SetSearcher(kernel, std::make_unique<DeterministicSearcher>());
+        DeterministicSearcher searcher;
         ConfigurationCount count(50);
-        CHECK(Simulate(configurations, manager.GetKernel(kernel), results, &count).size() == 50);
+        CHECK(Simulate(kernel, searcher, results, &count).size() == 50);
     }
     {
-        ConfigurationManager configurations;
-        configurations.SetSearcher(kernel, std::make_unique<DeterministicSearcher>());
+        DeterministicSearcher searcher;
         ConfigurationFraction fraction(0.1);
-        CHECK(Simulate(configurations, manager.GetKernel(kernel), results, &fraction).size() == 21);
+        CHECK(Simulate(kernel, searcher, results, &fraction).size() == 21);
     }
     {
         // The target is in milliseconds: stops at the first configuration of at most 1 ms.
-        ConfigurationManager configurations;
-        configurations.SetSearcher(kernel, std::make_unique<DeterministicSearcher>());
+        DeterministicSearcher searcher;
         ConfigurationDuration duration(1.0);
-        const auto order = Simulate(configurations, manager.GetKernel(kernel), results, &duration);
+        const auto order = Simulate(kernel, searcher, results, &duration);
         CHECK(!order.empty() && order.size() < 210);
-        size_t fast = 0;
-        for (const auto& result : results)
-        {
-            fast += result.GetTotalDuration() <= 1000000 ? 1 : 0;
-        }
-        CHECK(fast > 0);
-        ConfigurationManager again;
-        again.SetSearcher(kernel, std::make_unique<DeterministicSearcher>());
+        DeterministicSearcher again;
         ConfigurationDuration never(0.5);
-        CHECK(Simulate(again, manager.GetKernel(kernel), results, &never).size() == 210);
+        CHECK(Simulate(kernel, again, results, &never).size() == 210);
+    }
+    {
+        DeterministicSearcher searcher;
+        auto count = std::make_shared<ConfigurationCount>(30);
+        auto fraction = std::make_shared<ConfigurationFraction>(0.5);
+        UnionCondition either({count, fraction});
+        CHECK(Simulate(kernel, searcher, results, &either).size() == 30);
     }
 }
 
