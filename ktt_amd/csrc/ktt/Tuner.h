@@ -154,20 +154,22 @@ public:
 private:
     std::unique_ptr<TunerCore> m_Tuner;
 
-    ArgumentId AddArgumentWithReferencedData(const size_t elementSize, const ArgumentDataType dataType,
-        const ArgumentMemoryLocation memoryLocation, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-        const ArgumentManagementType managementType, void* data, const size_t dataSize, const ArgumentId& customId);
-    ArgumentId AddArgumentWithOwnedData(const size_t elementSize, const ArgumentDataType dataType,
-        const ArgumentMemoryLocation memoryLocation, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-        const ArgumentManagementType managementType, const void* data, const size_t dataSize, const ArgumentId& customId);
-    ArgumentId AddUserArgument(ComputeBuffer buffer, const size_t elementSize, const ArgumentDataType dataType,
-        const ArgumentMemoryLocation memoryLocation, const ArgumentAccessType accessType, const size_t dataSize,
-        const ArgumentId& customId);
-    void AddParameterInternal(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values,
+    /** Where a new argument's data comes from: a private copy, the caller's memory, or a device buffer. */
+    enum class DataSource
+    {
+        Copy,
+        Reference,
+        DeviceBuffer
+    };
+
+    ArgumentId RegisterArgument(const size_t elementSize, const ArgumentDataType dataType, const ArgumentMemoryLocation location,
+        const ArgumentAccessType access, const ArgumentMemoryType kind, const ArgumentManagementType management, const DataSource source,
+        void* data, const size_t bytes, const ArgumentId& name);
+    void AddParameterValues(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values,
         const std::string& group);
 
     template <typename T>
-    static ArgumentDataType DeriveArgumentDataType();
+    static ArgumentDataType ElementTypeOf();
 };
 
 } // namespace ktt

@@ -201,7 +201,7 @@ std::shared_ptr<HipKernel> CoulombRuntime::GetKernel(const std::string& file, co
     }
 
     const std::string path = m_KernelDirectory + "/" + file;
-    const std::string source = LoadFileToString(path);
+    const std::string source = ReadTextFile(path);
     std::vector<std::string> options = HipCompiler::DefaultOptions(m_Architecture);
     options.push_back("-I" + m_KernelDirectory);
     for (const auto& define : defines)
@@ -466,7 +466,7 @@ int ktt_hip_compile_check(const char* sourceFile, const char* kernelName, const 
     {
         path = CoulombRuntime::Get().GetKernelDirectory() + "/" + path;
     }
-    const std::string source = LoadFileToString(path);
+    const std::string source = ReadTextFile(path);
     const size_t slash = path.find_last_of('/');
     std::vector<std::string> options = HipCompiler::DefaultOptions("gfx950");
     options.push_back("-I" + (slash == std::string::npos ? std::string(".") : path.substr(0, slash)));

@@ -142,7 +142,7 @@ KernelResult RunRecord::Finish(const Nanoseconds launcherTime) const
 
 // ---------------------------------------------------------------- LaunchContext
 
-LaunchContext::LaunchContext(ComputeEngine& engine, KernelArgumentManager& arguments) :
+LaunchContext::LaunchContext(ComputeEngine& engine, ArgumentRegistry& arguments) :
     m_Engine(engine),
     m_Arguments(arguments),
     m_Record(nullptr)
@@ -234,7 +234,7 @@ KernelRunMode LaunchContext::GetRunMode() const { return Record().GetMode(); }
 
 void LaunchContext::ChangeArguments(const KernelDefinitionId id, const std::vector<ArgumentId>& arguments)
 {
-    Record().Rebind(id, m_Arguments.GetArguments(arguments));
+    Record().Rebind(id, m_Arguments.Resolve(arguments));
 }
 
 void LaunchContext::SwapArguments(const KernelDefinitionId id, const ArgumentId& first, const ArgumentId& second)
@@ -245,7 +245,7 @@ void LaunchContext::SwapArguments(const KernelDefinitionId id, const ArgumentId&
 void LaunchContext::UpdateScalarArgument(const ArgumentId& id, const void* data)
 {
     // The new value holds for this run only; the registered argument is left as it is.
-    const KernelArgument& original = m_Arguments.GetArgument(id);
+    const KernelArgument& original = m_Arguments.Get(id);
     if (original.GetMemoryType() != ArgumentMemoryType::Scalar)
     {
         throw KttException("Argument " + id + " is not a scalar");
@@ -257,7 +257,7 @@ void LaunchContext::UpdateScalarArgument(const ArgumentId& id, const void* data)
 
 void LaunchContext::UpdateLocalArgument(const ArgumentId& id, const size_t dataSize)
 {
-    const KernelArgument& original = m_Arguments.GetArgument(id);
+    const KernelArgument& original = m_Arguments.Get(id);
     if (original.GetMemoryType() != ArgumentMemoryType::Local)
     {
         throw KttException("Argument " + id + " is not a local-memory argument");
@@ -274,7 +274,7 @@ void LaunchContext::UploadBuffer(const ArgumentId& id)
 
 TransferActionId LaunchContext::UploadBufferAsync(const ArgumentId& id, const QueueId queue)
 {
-    return m_Engine.UploadArgument(m_Arguments.GetArgument(id), queue);
+    return m_Engine.UploadArgument(m_Arguments.Get(id), queue);
 }
 
 void LaunchContext::DownloadBuffer(const ArgumentId& id, void* destination, const size_t dataSize)
@@ -664,7 +664,7 @@ ResultStatus StatusFor(const ExceptionReason reason)
 
 } // namespace
 
-Executor::Executor(ComputeEngine& engine, KernelArgumentManager& arguments) :
+Executor::Executor(ComputeEngine& engine, ArgumentRegistry& arguments) :
     m_Engine(engine),
     m_Arguments(arguments),
     m_Context(std::make_unique<LaunchContext>(engine, arguments)),

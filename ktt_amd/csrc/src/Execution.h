@@ -67,7 +67,7 @@ private:
 class LaunchContext : public ComputeInterface
 {
 public:
-    LaunchContext(ComputeEngine& engine, KernelArgumentManager& arguments);
+    LaunchContext(ComputeEngine& engine, ArgumentRegistry& arguments);
 
     void Begin(RunRecord& record) { m_Record = &record; }
     void End() { m_Record = nullptr; }
@@ -113,7 +113,7 @@ public:
 
 private:
     ComputeEngine& m_Engine;
-    KernelArgumentManager& m_Arguments;
+    ArgumentRegistry& m_Arguments;
     RunRecord* m_Record;
 
     RunRecord& Record() const;
@@ -176,7 +176,7 @@ private:
 class Executor
 {
 public:
-    Executor(ComputeEngine& engine, KernelArgumentManager& arguments);
+    Executor(ComputeEngine& engine, ArgumentRegistry& arguments);
 
     KernelResult Execute(const Kernel& kernel, const KernelConfiguration& configuration, const KernelDimensions& dimensions,
         const KernelRunMode mode, const std::vector<BufferOutputDescriptor>& output, const bool manageBuffers = true);
@@ -186,12 +186,12 @@ public:
     void SetProfiling(const bool flag) { m_Engine.SetProfiling(flag); }
     bool IsProfiling() const { return m_Engine.IsProfilingActive(); }
     OutputChecker& Checker() { return *m_Checker; }
-    KernelArgumentManager& Arguments() { return m_Arguments; }
+    ArgumentRegistry& Arguments() { return m_Arguments; }
     void ForgetKernel(const KernelId id) { m_Checker->ForgetReferenceKernel(id); }
 
 private:
     ComputeEngine& m_Engine;
-    KernelArgumentManager& m_Arguments;
+    ArgumentRegistry& m_Arguments;
     std::unique_ptr<LaunchContext> m_Context;
     std::unique_ptr<OutputChecker> m_Checker;
     bool m_KeepReadOnly;

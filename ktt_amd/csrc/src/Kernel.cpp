@@ -1,219 +1,133 @@
+#include <algorithm>
 #include <fstream>
-#include <sstream>
+#include <iterator>
 
 #include <KttException.h>
 
-#include "Common.h"
 #include "Kernel.h"
 
 namespace ktt
 {
 
-std::string LoadFileToString(const std::string& filePath)
+std::string ReadTextFile(const std::string& path)
 {
-    std::ifstream file(filePath, std::ios::binary);
-    if (!file.is_open())
+    std::ifstream file(path, std::ios::binary);
+    if (!file)
     {
-        throw KttException("Unable to open file: " + filePath);
+        throw KttException("Cannot read " + path);
     }
-    std::stringstream stream;
-    stream << file.rdbuf();
-    return stream.str();
+    return std::string(std::istreambuf_iterator<char>(file), std::istreambuf_iterator<char>());
 }
 
-// ---------------------------------------------------------------- KernelParameter
+// ---------------------------------------------------------------- parameters and constraints
 
 KernelParameter::KernelParameter(const std::string& name, const std::vector<ParameterValue>& values, const std::string& group) :
     m_Name(name),
+    m_Group(group),
     m_Values(values),
-    m_Group(group)
+    m_Type(ParameterValueType::UnsignedInt)
 {
     if (values.empty())
     {
-        throw KttException("Kernel parameter with name " + name + " must have at least one value");
+        throw KttException("Parameter " + name + " needs at least one value");
     }
-    const auto type = ParameterPair::GetTypeFromValue(values[0]);
-    for (const auto& value : values)
+    m_Type = ParameterPair::GetTypeFromValue(values.front());
+    const bool mixed = std::any_of(values.begin(), values.end(),
+        [this](const ParameterValue& value) { return ParameterPair::GetTypeFromValue(value) != m_Type; });
+    if (mixed)
     {
-        if (ParameterPair::GetTypeFromValue(value) != type)
-        {
-            throw KttException("All values of kernel parameter with name " + name + " must have the same type");
-        }
+        throw KttException("The values of parameter " + name + " must all have one type");
     }
 }
-
-ParameterValueType KernelParameter::GetValueType() const
-{
-    return ParameterPair::GetTypeFromValue(m_Values[0]);
-}
-
-ParameterPair KernelParameter::GeneratePair(const size_t valueIndex) const
-{
-    return ParameterPair(m_Name, m_Values.at(valueIndex));
-}
-
-std::vector<ParameterPair> KernelParameter::GeneratePairs() const
-{
-    std::vector<ParameterPair> result;
-    for (size_t i = 0; i < m_Values.size(); ++i)
-    {
-        result.push_back(GeneratePair(i));
-    }
-    return result;
-}
-
-// ---------------------------------------------------------------- KernelConstraint
 
 KernelConstraint::KernelConstraint(const std::vector<std::string>& parameters, ConstraintFunction function) :
     m_Parameters(parameters),
-    m_Function(function),
-    m_GenericFunction(nullptr)
+    m_OnIntegers(function)
 {
     if (!function)
     {
-        throw KttException("Constraint function must be valid");
+        throw KttException("A constraint needs a callable function");
     }
 }
 
 KernelConstraint::KernelConstraint(const std::vector<std::string>& parameters, GenericConstraintFunction function) :
     m_Parameters(parameters),
-    m_Function(nullptr),
-    m_GenericFunction(function)
+    m_OnValues(function)
 {
     if (!function)
     {
-        throw KttException("Constraint function must be valid");
+        throw KttException("A constraint needs a callable function");
     }
-}
-
-bool KernelConstraint::HasAllParameters(const std::vector<std::string>& parameters) const
-{
-    for (const auto& name : m_Parameters)
-    {
-        if (!ContainsElement(parameters, name))
-        {
-            return false;
-        }
-    }
-    return true;
 }
 
 bool KernelConstraint::IsFulfilled(const std::vector<const ParameterValue*>& values) const
 {
-    if (m_GenericFunction)
+    if (m_OnValues)
     {
-        return m_GenericFunction(values);
+        return m_OnValues(values);
     }
-
+    // Integer constraints see every numeric or boolean value as an unsigned integer.
     std::vector<uint64_t> integers;
     integers.reserve(values.size());
-    for (const auto* value : values)
+    for (const ParameterValue* value : values)
     {
-        if (std::holds_alternative<uint64_t>(*value))
+        integers.push_back(std::visit([](const auto& held) -> uint64_t
         {
-            integers.push_back(std::get<uint64_t>(*value));
-        }
-        else if (std::holds_alternative<int64_t>(*value))
-        {
-            integers.push_back(static_cast<uint64_t>(std::get<int64_t>(*value)));
-        }
-        else if (std::holds_alternative<bool>(*value))
-        {
-            integers.push_back(std::get<bool>(*value) ? 1 : 0);
-        }
-        else if (std::holds_alternative<double>(*value))
-        {
-            integers.push_back(static_cast<uint64_t>(std::get<double>(*value)));
-        }
-        else
-        {
-            throw KttException("Constraints over string parameters require a generic constraint");
-        }
+            using T = std::decay_t<decltype(held)>;
+            if constexpr (std::is_same_v<T, std::string>)
+            {
+                throw KttException("A constraint over string values must be a generic constraint");
+            }
+            else
+            {
+                return static_cast<uint64_t>(held);
+            }
+        }, *value));
     }
-    return m_Function(integers);
-}
-
-// ---------------------------------------------------------------- ThreadModifier
-
-ThreadModifier::ThreadModifier(const std::vector<std::string>& parameters, const std::vector<KernelDefinitionId>& definitions,
-    ModifierFunction function) :
-    m_Parameters(parameters),
-    m_Definitions(definitions),
-    m_Function(function)
-{
-    if (!function)
-    {
-        throw KttException("Thread modifier function must be valid");
-    }
-}
-
-uint64_t ThreadModifier::GetModifiedSize(const KernelDefinitionId id, const uint64_t originalSize,
-    const std::vector<ParameterPair>& pairs) const
-{
-    if (!ContainsElement(m_Definitions, id))
-    {
-        return originalSize;
-    }
-    const std::vector<uint64_t> values = ParameterPair::GetParameterValues<uint64_t>(pairs, m_Parameters);
-    return m_Function(originalSize, values);
+    return m_OnIntegers(integers);
 }
 
 // ---------------------------------------------------------------- KernelDefinition
 
-KernelDefinition::KernelDefinition(const KernelDefinitionId id, const std::string& name, const std::string& source,
-    const DimensionVector& globalSize, const DimensionVector& localSize, const std::vector<std::string>& typeNames,
-    const std::string& sourceDirectory) :
+KernelDefinition::KernelDefinition(const KernelDefinitionId id, const std::string& name, const std::vector<std::string>& typeNames,
+    const std::string& source, const std::string& sourceDirectory, const DimensionVector& globalSize, const DimensionVector& localSize) :
     m_Id(id),
     m_Name(name),
+    m_Templated(TemplatedName(name, typeNames)),
     m_Source(source),
-    m_TemplatedName(CreateTemplatedName(name, typeNames)),
-    m_TypeNames(typeNames),
-    m_GlobalSize(globalSize),
-    m_LocalSize(localSize),
-    m_SourceDirectory(sourceDirectory)
+    m_Directory(sourceDirectory),
+    m_Global(globalSize),
+    m_Local(localSize)
 {}
 
-void KernelDefinition::SetArguments(const std::vector<KernelArgument*> arguments)
+void KernelDefinition::SetArguments(const std::vector<KernelArgument*>& arguments)
 {
-    for (size_t i = 0; i < arguments.size(); ++i)
+    std::vector<const KernelArgument*> sorted(arguments.begin(), arguments.end());
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
     {
-        for (size_t j = i + 1; j < arguments.size(); ++j)
-        {
-            if (arguments[i] == arguments[j])
-            {
-                throw KttException("Kernel arguments for a single kernel definition must be unique");
-            }
-        }
+        throw KttException("Definition " + m_Templated + " binds the same argument twice; pass each argument once");
     }
     m_Arguments = arguments;
 }
 
-std::vector<KernelArgument*> KernelDefinition::GetVectorArguments() const
-{
-    return KernelArgument::GetArgumentsWithMemoryType(m_Arguments, ArgumentMemoryType::Vector);
-}
-
 bool KernelDefinition::HasArgument(const ArgumentId& id) const
 {
-    return ContainsElementIf(m_Arguments, [&id](const auto* argument) { return argument->GetId() == id; });
+    return std::any_of(m_Arguments.begin(), m_Arguments.end(), [&id](const KernelArgument* argument) { return argument->GetId() == id; });
 }
 
-std::string KernelDefinition::CreateTemplatedName(const std::string& name, const std::vector<std::string>& typeNames)
+std::string KernelDefinition::TemplatedName(const std::string& name, const std::vector<std::string>& typeNames)
 {
     if (typeNames.empty())
     {
         return name;
     }
-    std::string result = name + "<";
+    std::string text = name + "<";
     for (size_t i = 0; i < typeNames.size(); ++i)
     {
-        result += typeNames[i];
-        if (i + 1 < typeNames.size())
-        {
-            result += ", ";
-        }
+        text += (i == 0 ? "" : ", ") + typeNames[i];
     }
-    return result + ">";
+    return text + ">";
 }
 
 // ---------------------------------------------------------------- Kernel
@@ -221,229 +135,210 @@ std::string KernelDefinition::CreateTemplatedName(const std::string& name, const
 Kernel::Kernel(const KernelId id, const std::string& name, const std::vector<const KernelDefinition*>& definitions) :
     m_Id(id),
     m_Name(name),
-    m_Definitions(definitions),
-    m_ProfiledDefinitions(definitions.empty() ? std::vector<const KernelDefinition*>{} : std::vector<const KernelDefinition*>{definitions[0]})
+    m_Definitions(definitions)
 {
     if (definitions.empty())
     {
-        throw KttException("Kernel must have at least one definition");
+        throw KttException("Kernel " + name + " needs at least one definition");
     }
+    // Only the first definition is profiled unless the caller chooses otherwise.
+    m_Profiled = {definitions.front()};
+}
+
+const KernelParameter* Kernel::FindParameter(const std::string& name) const
+{
+    const auto found = std::find_if(m_Parameters.begin(), m_Parameters.end(),
+        [&name](const KernelParameter& parameter) { return parameter.GetName() == name; });
+    return found != m_Parameters.end() ? &*found : nullptr;
+}
+
+const KernelParameter& Kernel::GetParameter(const std::string& name) const
+{
+    const KernelParameter* parameter = FindParameter(name);
+    if (parameter == nullptr)
+    {
+        throw KttException("Kernel " + m_Name + " has no parameter " + name);
+    }
+    return *parameter;
 }
 
 void Kernel::AddParameter(const KernelParameter& parameter)
 {
     if (HasParameter(parameter.GetName()))
     {
-        throw KttException("Kernel parameter with name " + parameter.GetName() + " already exists");
+        throw KttException("Kernel " + m_Name + " already has a parameter " + parameter.GetName());
     }
     m_Parameters.push_back(parameter);
 }
 
 void Kernel::AddConstraint(const KernelConstraint& constraint)
 {
-    std::string group;
-    bool first = true;
-    for (const auto& name : constraint.GetParameters())
+    const auto& names = constraint.GetParameters();
+    if (names.empty())
     {
-        const auto& parameter = GetParameter(name);
-        if (!first && parameter.GetGroup() != group)
+        throw KttException("A constraint of kernel " + m_Name + " names no parameter");
+    }
+    const std::string& group = GetParameter(names.front()).GetGroup();
+    for (const std::string& name : names)
+    {
+        if (GetParameter(name).GetGroup() != group)
         {
-            throw KttException("Constraint can only be added between parameters that belong to the same group");
+            throw KttException("The parameters of a constraint must belong to one group (kernel " + m_Name + ")");
         }
-        group = parameter.GetGroup();
-        first = false;
     }
     m_Constraints.push_back(constraint);
 }
 
-void Kernel::AddThreadModifier(const ModifierType type, const ModifierDimension dimension, const ThreadModifier& modifier)
+void Kernel::AddThreadModifier(const ModifierType type, const ModifierDimension dimension, const std::vector<std::string>& parameters,
+    const std::vector<KernelDefinitionId>& definitions, ModifierFunction function)
 {
-    for (const auto& name : modifier.GetParameters())
+    if (!function)
     {
-        const auto& parameter = GetParameter(name);
-        const auto valueType = parameter.GetValueType();
+        throw KttException("A thread modifier needs a callable function");
+    }
+    for (const std::string& name : parameters)
+    {
+        const ParameterValueType valueType = GetParameter(name).GetValueType();
         if (valueType != ParameterValueType::UnsignedInt && valueType != ParameterValueType::Int)
         {
-            throw KttException("Parameter with name " + name + " used by a thread modifier must have integer values");
+            throw KttException("Thread modifiers take integer parameters only; " + name + " is not one");
         }
     }
-    for (const auto id : modifier.GetDefinitions())
+    for (const KernelDefinitionId id : definitions)
     {
         if (!HasDefinition(id))
         {
-            throw KttException("Kernel with id " + std::to_string(m_Id) + " does not contain definition with id "
-                + std::to_string(id));
+            throw KttException("Kernel " + m_Name + " has no definition with id " + std::to_string(id));
         }
     }
-    m_Modifiers[type][dimension].push_back(modifier);
-}
-
-void Kernel::SetLauncher(KernelLauncher launcher)
-{
-    m_Launcher = launcher;
+    m_Modifiers.push_back({type, dimension, parameters, definitions, function});
 }
 
 void Kernel::SetProfiledDefinitions(const std::vector<const KernelDefinition*>& definitions)
 {
-    for (const auto* definition : definitions)
+    for (const KernelDefinition* definition : definitions)
     {
         if (!HasDefinition(definition->GetId()))
         {
-            throw KttException("Kernel with id " + std::to_string(m_Id) + " does not contain definition with id "
-                + std::to_string(definition->GetId()));
+            throw KttException("Kernel " + m_Name + " has no definition " + definition->GetTemplatedName());
         }
     }
-    m_ProfiledDefinitions = definitions;
-}
-
-const KernelDefinition& Kernel::GetPrimaryDefinition() const
-{
-    return *m_Definitions[0];
-}
-
-const KernelDefinition& Kernel::GetDefinition(const KernelDefinitionId id) const
-{
-    for (const auto* definition : m_Definitions)
-    {
-        if (definition->GetId() == id)
-        {
-            return *definition;
-        }
-    }
-    throw KttException("Kernel with id " + std::to_string(m_Id) + " does not contain definition with id " + std::to_string(id));
-}
-
-const KernelParameter& Kernel::GetParameter(const std::string& name) const
-{
-    for (const auto& parameter : m_Parameters)
-    {
-        if (parameter.GetName() == name)
-        {
-            return parameter;
-        }
-    }
-    throw KttException("Kernel parameter with name " + name + " does not exist");
-}
-
-std::vector<KernelArgument*> Kernel::GetVectorArguments() const
-{
-    std::vector<KernelArgument*> result;
-    for (const auto* definition : m_Definitions)
-    {
-        for (auto* argument : definition->GetVectorArguments())
-        {
-            if (!ContainsElement(result, argument))
-            {
-                result.push_back(argument);
-            }
-        }
-    }
-    return result;
+    m_Profiled = definitions;
 }
 
 bool Kernel::HasDefinition(const KernelDefinitionId id) const
 {
-    return ContainsElementIf(m_Definitions, [id](const auto* definition) { return definition->GetId() == id; });
+    return std::any_of(m_Definitions.begin(), m_Definitions.end(), [id](const KernelDefinition* definition) { return definition->GetId() == id; });
 }
 
-bool Kernel::HasParameter(const std::string& name) const
+const KernelDefinition& Kernel::GetDefinition(const KernelDefinitionId id) const
 {
-    return ContainsElementIf(m_Parameters, [&name](const auto& parameter) { return parameter.GetName() == name; });
+    const auto found = std::find_if(m_Definitions.begin(), m_Definitions.end(),
+        [id](const KernelDefinition* definition) { return definition->GetId() == id; });
+    if (found == m_Definitions.end())
+    {
+        throw KttException("Kernel " + m_Name + " has no definition with id " + std::to_string(id));
+    }
+    return **found;
+}
+
+std::vector<KernelArgument*> Kernel::GetVectorArguments() const
+{
+    std::vector<KernelArgument*> vectors;
+    for (const KernelDefinition* definition : m_Definitions)
+    {
+        for (KernelArgument* argument : definition->GetVectorArguments())
+        {
+            if (std::find(vectors.begin(), vectors.end(), argument) == vectors.end())
+            {
+                vectors.push_back(argument);
+            }
+        }
+    }
+    return vectors;
 }
 
 KernelConfiguration Kernel::CreateConfiguration(const ParameterInput& parameters) const
 {
     std::vector<ParameterPair> pairs;
-    for (const auto& [name, value] : parameters)
+    for (const KernelParameter& parameter : m_Parameters)
     {
-        const auto& parameter = GetParameter(name);
-        if (ParameterPair::GetTypeFromValue(value) != parameter.GetValueType())
-        {
-            throw KttException("Value type mismatch for parameter with name " + name);
-        }
-        pairs.emplace_back(name, value);
-    }
-    for (const auto& parameter : m_Parameters)
-    {
-        const bool included = ContainsElementIf(pairs, [&parameter](const auto& pair) { return pair.GetName() == parameter.GetName(); });
-        if (!included)
+        const auto given = std::find_if(parameters.begin(), parameters.end(),
+            [&parameter](const auto& entry) { return entry.first == parameter.GetName(); });
+        if (given == parameters.end())
         {
             pairs.push_back(parameter.GeneratePair(0));
+            continue;
+        }
+        if (ParameterPair::GetTypeFromValue(given->second) != parameter.GetValueType())
+        {
+            throw KttException("The value given for " + parameter.GetName() + " has the wrong type");
+        }
+        pairs.emplace_back(parameter.GetName(), given->second);
+    }
+    for (const auto& entry : parameters)
+    {
+        if (!HasParameter(entry.first))
+        {
+            throw KttException("Kernel " + m_Name + " has no parameter " + entry.first);
         }
     }
     return KernelConfiguration(pairs);
 }
 
-DimensionVector Kernel::GetModifiedSize(const KernelDefinitionId id, const ModifierType type, const std::vector<ParameterPair>& pairs) const
-{
-    const auto& definition = GetDefinition(id);
-    const auto& defaultSize = type == ModifierType::Global ? definition.GetGlobalSize() : definition.GetLocalSize();
-    return GetModifiedSize(id, defaultSize, type, pairs);
-}
-
-DimensionVector Kernel::GetModifiedSize(const KernelDefinitionId id, const DimensionVector& originalSize, const ModifierType type,
+DimensionVector Kernel::GetModifiedSize(const KernelDefinitionId id, const DimensionVector& size, const ModifierType type,
     const std::vector<ParameterPair>& pairs) const
 {
-    auto typeIterator = m_Modifiers.find(type);
-    if (typeIterator == m_Modifiers.cend())
+    DimensionVector modified = size;
+    for (const Modifier& modifier : m_Modifiers)
     {
-        return originalSize;
-    }
-
-    DimensionVector result;
-    for (int i = 0; i <= static_cast<int>(ModifierDimension::Z); ++i)
-    {
-        const auto dimension = static_cast<ModifierDimension>(i);
-        uint64_t size = originalSize.GetSize(dimension);
-        auto dimensionIterator = typeIterator->second.find(dimension);
-        if (dimensionIterator != typeIterator->second.cend())
+        const bool applies = modifier.type == type
+            && std::find(modifier.definitions.begin(), modifier.definitions.end(), id) != modifier.definitions.end();
+        if (!applies)
         {
-            for (const auto& modifier : dimensionIterator->second)
-            {
-                size = modifier.GetModifiedSize(id, size, pairs);
-            }
+            continue;
         }
-        result.SetSize(dimension, static_cast<size_t>(size));
+        const std::vector<uint64_t> values = ParameterPair::GetParameterValues<uint64_t>(pairs, modifier.parameters);
+        const uint64_t before = modified.GetSize(modifier.dimension);
+        modified.SetSize(modifier.dimension, static_cast<size_t>(modifier.function(before, values)));
     }
-    return result;
+    return modified;
 }
 
-// ---------------------------------------------------------------- KernelManager
+// ---------------------------------------------------------------- KernelRegistry
 
-KernelManager::KernelManager(KernelArgumentManager& argumentManager) :
-    m_ArgumentManager(argumentManager),
-    m_NextDefinitionId(0),
-    m_NextKernelId(0)
+KernelRegistry::KernelRegistry(ArgumentRegistry& arguments) :
+    m_Arguments(arguments),
+    m_NextDefinition(0),
+    m_NextKernel(0)
 {}
 
-KernelDefinitionId KernelManager::AddKernelDefinition(const std::string& name, const std::string& source,
-    const DimensionVector& globalSize, const DimensionVector& localSize, const std::vector<std::string>& typeNames,
-    const std::string& sourceDirectory)
+KernelDefinitionId KernelRegistry::AddDefinition(const std::string& name, const std::string& source, const DimensionVector& globalSize,
+    const DimensionVector& localSize, const std::vector<std::string>& typeNames, const std::string& sourceDirectory)
 {
-    if (GetDefinitionId(name, typeNames) != InvalidKernelDefinitionId)
+    if (FindDefinition(name, typeNames) != InvalidKernelDefinitionId)
     {
-        throw KttException("Kernel definition with name " + name + " already exists");
+        throw KttException("A definition " + KernelDefinition::TemplatedName(name, typeNames) + " exists already");
     }
-    const KernelDefinitionId id = m_NextDefinitionId++;
-    m_Definitions[id] = std::make_unique<KernelDefinition>(id, name, source, globalSize, localSize, typeNames, sourceDirectory);
+    const KernelDefinitionId id = m_NextDefinition++;
+    m_Definitions.emplace(id, std::make_unique<KernelDefinition>(id, name, typeNames, source, sourceDirectory, globalSize, localSize));
     return id;
 }
 
-KernelDefinitionId KernelManager::AddKernelDefinitionFromFile(const std::string& name, const std::string& filePath,
+KernelDefinitionId KernelRegistry::AddDefinitionFromFile(const std::string& name, const std::string& filePath,
     const DimensionVector& globalSize, const DimensionVector& localSize, const std::vector<std::string>& typeNames)
 {
-    const std::string source = LoadFileToString(filePath);
-    const size_t slash = filePath.find_last_of('/');
-    const std::string directory = slash == std::string::npos ? std::string(".") : filePath.substr(0, slash);
-    return AddKernelDefinition(name, source, globalSize, localSize, typeNames, directory);
+    const size_t slash = filePath.rfind('/');
+    const std::string directory = slash == std::string::npos ? "." : filePath.substr(0, slash);
+    return AddDefinition(name, ReadTextFile(filePath), globalSize, localSize, typeNames, directory);
 }
 
-KernelDefinitionId KernelManager::GetDefinitionId(const std::string& name, const std::vector<std::string>& typeNames) const
+KernelDefinitionId KernelRegistry::FindDefinition(const std::string& name, const std::vector<std::string>& typeNames) const
 {
-    const std::string templatedName = KernelDefinition::CreateTemplatedName(name, typeNames);
+    const std::string wanted = KernelDefinition::TemplatedName(name, typeNames);
     for (const auto& [id, definition] : m_Definitions)
     {
-        if (definition->GetTemplatedName() == templatedName)
+        if (definition->GetTemplatedName() == wanted)
         {
             return id;
         }
@@ -451,153 +346,87 @@ KernelDefinitionId KernelManager::GetDefinitionId(const std::string& name, const
     return InvalidKernelDefinitionId;
 }
 
-void KernelManager::RemoveKernelDefinition(const KernelDefinitionId id)
+void KernelRegistry::RemoveDefinition(const KernelDefinitionId id)
 {
-    for (const auto& [kernelId, kernel] : m_Kernels)
+    const bool inUse = std::any_of(m_Kernels.begin(), m_Kernels.end(), [id](const auto& entry) { return entry.second->HasDefinition(id); });
+    if (inUse)
     {
-        if (kernel->HasDefinition(id))
-        {
-            throw KttException("Kernel definition with id " + std::to_string(id)
-                + " cannot be removed because it is still referenced by kernel with id " + std::to_string(kernelId));
-        }
+        throw KttException("Definition " + std::to_string(id) + " belongs to a kernel; remove the kernel first");
     }
     if (m_Definitions.erase(id) == 0)
     {
-        throw KttException("Attempting to remove kernel definition with invalid id: " + std::to_string(id));
+        throw KttException("No definition with id " + std::to_string(id));
     }
 }
 
-void KernelManager::SetArguments(const KernelDefinitionId id, const std::vector<ArgumentId>& argumentIds)
+void KernelRegistry::Bind(const KernelDefinitionId id, const std::vector<ArgumentId>& arguments)
 {
-    auto& definition = GetDefinition(id);
-    definition.SetArguments(m_ArgumentManager.GetArguments(argumentIds));
+    GetDefinition(id).SetArguments(m_Arguments.Resolve(arguments));
 }
 
-KernelId KernelManager::CreateKernel(const std::string& name, const std::vector<KernelDefinitionId>& definitionIds)
+std::vector<const KernelDefinition*> KernelRegistry::Definitions(const std::vector<KernelDefinitionId>& ids) const
 {
-    for (const auto& [id, kernel] : m_Kernels)
-    {
-        if (kernel->GetName() == name)
-        {
-            throw KttException("Kernel with name " + name + " already exists");
-        }
-    }
     std::vector<const KernelDefinition*> definitions;
-    for (const auto definitionId : definitionIds)
+    for (const KernelDefinitionId id : ids)
     {
-        definitions.push_back(&GetDefinition(definitionId));
+        definitions.push_back(&GetDefinition(id));
     }
-    const KernelId id = m_NextKernelId++;
-    m_Kernels[id] = std::make_unique<Kernel>(id, name, definitions);
+    return definitions;
+}
+
+KernelId KernelRegistry::AddKernel(const std::string& name, const std::vector<KernelDefinitionId>& definitions)
+{
+    const bool taken = std::any_of(m_Kernels.begin(), m_Kernels.end(), [&name](const auto& entry) { return entry.second->GetName() == name; });
+    if (taken)
+    {
+        throw KttException("A kernel named " + name + " exists already");
+    }
+    const KernelId id = m_NextKernel++;
+    m_Kernels.emplace(id, std::make_unique<Kernel>(id, name, Definitions(definitions)));
     return id;
 }
 
-void KernelManager::RemoveKernel(const KernelId id)
+void KernelRegistry::RemoveKernel(const KernelId id)
 {
     if (m_Kernels.erase(id) == 0)
     {
-        throw KttException("Attempting to remove kernel with invalid id: " + std::to_string(id));
+        throw KttException("No kernel with id " + std::to_string(id));
     }
 }
 
-void KernelManager::SetLauncher(const KernelId id, KernelLauncher launcher)
+Kernel& KernelRegistry::GetKernel(const KernelId id)
 {
-    GetKernel(id).SetLauncher(launcher);
+    return const_cast<Kernel&>(static_cast<const KernelRegistry*>(this)->GetKernel(id));
 }
 
-void KernelManager::AddParameter(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values,
-    const std::string& group)
+const Kernel& KernelRegistry::GetKernel(const KernelId id) const
 {
-    GetKernel(id).AddParameter(KernelParameter(name, values, group));
-}
-
-void KernelManager::AddConstraint(const KernelId id, const std::vector<std::string>& parameters, ConstraintFunction function)
-{
-    GetKernel(id).AddConstraint(KernelConstraint(parameters, function));
-}
-
-void KernelManager::AddGenericConstraint(const KernelId id, const std::vector<std::string>& parameters,
-    GenericConstraintFunction function)
-{
-    GetKernel(id).AddConstraint(KernelConstraint(parameters, function));
-}
-
-void KernelManager::AddThreadModifier(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds,
-    const ModifierType type, const ModifierDimension dimension, const std::vector<std::string>& parameters, ModifierFunction function)
-{
-    GetKernel(id).AddThreadModifier(type, dimension, ThreadModifier(parameters, definitionIds, function));
-}
-
-void KernelManager::SetProfiledDefinitions(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds)
-{
-    std::vector<const KernelDefinition*> definitions;
-    for (const auto definitionId : definitionIds)
+    const auto found = m_Kernels.find(id);
+    if (found == m_Kernels.end())
     {
-        definitions.push_back(&GetDefinition(definitionId));
+        throw KttException("No kernel with id " + std::to_string(id));
     }
-    GetKernel(id).SetProfiledDefinitions(definitions);
+    return *found->second;
 }
 
-Kernel& KernelManager::GetKernel(const KernelId id)
+KernelDefinition& KernelRegistry::GetDefinition(const KernelDefinitionId id)
 {
-    auto iterator = m_Kernels.find(id);
-    if (iterator == m_Kernels.end())
+    return const_cast<KernelDefinition&>(static_cast<const KernelRegistry*>(this)->GetDefinition(id));
+}
+
+const KernelDefinition& KernelRegistry::GetDefinition(const KernelDefinitionId id) const
+{
+    const auto found = m_Definitions.find(id);
+    if (found == m_Definitions.end())
     {
-        throw KttException("Attempting to retrieve kernel with invalid id: " + std::to_string(id));
+        throw KttException("No definition with id " + std::to_string(id));
     }
-    return *iterator->second;
+    return *found->second;
 }
 
-const Kernel& KernelManager::GetKernel(const KernelId id) const
+bool KernelRegistry::UsesArgument(const ArgumentId& id) const
 {
-    auto iterator = m_Kernels.find(id);
-    if (iterator == m_Kernels.end())
-    {
-        throw KttException("Attempting to retrieve kernel with invalid id: " + std::to_string(id));
-    }
-    return *iterator->second;
-}
-
-KernelDefinition& KernelManager::GetDefinition(const KernelDefinitionId id)
-{
-    auto iterator = m_Definitions.find(id);
-    if (iterator == m_Definitions.end())
-    {
-        throw KttException("Attempting to retrieve kernel definition with invalid id: " + std::to_string(id));
-    }
-    return *iterator->second;
-}
-
-const KernelDefinition& KernelManager::GetDefinition(const KernelDefinitionId id) const
-{
-    auto iterator = m_Definitions.find(id);
-    if (iterator == m_Definitions.end())
-    {
-        throw KttException("Attempting to retrieve kernel definition with invalid id: " + std::to_string(id));
-    }
-    return *iterator->second;
-}
-
-bool KernelManager::HasKernel(const KernelId id) const
-{
-    return ContainsKey(m_Kernels, id);
-}
-
-bool KernelManager::HasDefinition(const KernelDefinitionId id) const
-{
-    return ContainsKey(m_Definitions, id);
-}
-
-bool KernelManager::IsArgumentUsed(const ArgumentId& id) const
-{
-    for (const auto& [definitionId, definition] : m_Definitions)
-    {
-        if (definition->HasArgument(id))
-        {
-            return true;
-        }
-    }
-    return false;
+    return std::any_of(m_Definitions.begin(), m_Definitions.end(), [&id](const auto& entry) { return entry.second->HasArgument(id); });
 }
 
 } // namespace ktt

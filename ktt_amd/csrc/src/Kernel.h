@@ -1,7 +1,13 @@
-// Kernel model: definitions (source + default sizes + arguments), kernels (one or more
-// definitions, tuning parameters, constraints, thread modifiers, launcher) and the
-// kernel manager (Source/Kernel/*.cpp of the reference). Thread modifiers are applied
-// per dimension in the order they were added (Kernel.cpp:266-308, ThreadModifier.cpp:44-59).
+// Kernels and their registry.
+//
+//   KernelDefinition  one kernel function: name (with template arguments), source, default
+//                     global and local sizes, bound arguments.
+//   Kernel            what is tuned: one or more definitions, tuning parameters (with groups),
+//                     constraints, thread modifiers and an optional launcher.
+//   KernelRegistry    owns definitions and kernels and hands out their ids.
+//
+// A thread modifier maps a size along one dimension through a function of parameter values;
+// the modifiers of a dimension apply in the order they were added.
 #pragma once
 
 #include <cstdint>
@@ -22,173 +28,168 @@ namespace ktt
 class KernelParameter
 {
 public:
-    explicit KernelParameter(const std::string& name, const std::vector<ParameterValue>& values, const std::string& group);
+    KernelParameter(const std::string& name, const std::vector<ParameterValue>& values, const std::string& group);
 
     const std::string& GetName() const { return m_Name; }
     const std::string& GetGroup() const { return m_Group; }
     const std::vector<ParameterValue>& GetValues() const { return m_Values; }
     size_t GetValuesCount() const { return m_Values.size(); }
-    ParameterValueType GetValueType() const;
-    ParameterPair GeneratePair(const size_t valueIndex) const;
-    std::vector<ParameterPair> GeneratePairs() const;
+    ParameterValueType GetValueType() const { return m_Type; }
+    /** The pair (name, values[index]). */
+    ParameterPair GeneratePair(const size_t index) const { return ParameterPair(m_Name, m_Values.at(index)); }
 
 private:
     std::string m_Name;
-    std::vector<ParameterValue> m_Values;
     std::string m_Group;
+    std::vector<ParameterValue> m_Values;
+    ParameterValueType m_Type;
 };
 
 class KernelConstraint
 {
 public:
-    explicit KernelConstraint(const std::vector<std::string>& parameters, ConstraintFunction function);
-    explicit KernelConstraint(const std::vector<std::string>& parameters, GenericConstraintFunction function);
+    KernelConstraint(const std::vector<std::string>& parameters, ConstraintFunction function);
+    KernelConstraint(const std::vector<std::string>& parameters, GenericConstraintFunction function);
 
     const std::vector<std::string>& GetParameters() const { return m_Parameters; }
-    bool HasAllParameters(const std::vector<std::string>& parameters) const;
+    /** Evaluates the constraint on values given in the order of GetParameters(). */
     bool IsFulfilled(const std::vector<const ParameterValue*>& values) const;
 
 private:
     std::vector<std::string> m_Parameters;
-    ConstraintFunction m_Function;
-    GenericConstraintFunction m_GenericFunction;
-};
-
-class ThreadModifier
-{
-public:
-    explicit ThreadModifier(const std::vector<std::string>& parameters, const std::vector<KernelDefinitionId>& definitions,
-        ModifierFunction function);
-
-    const std::vector<std::string>& GetParameters() const { return m_Parameters; }
-    const std::vector<KernelDefinitionId>& GetDefinitions() const { return m_Definitions; }
-    uint64_t GetModifiedSize(const KernelDefinitionId id, const uint64_t originalSize, const std::vector<ParameterPair>& pairs) const;
-
-private:
-    std::vector<std::string> m_Parameters;
-    std::vector<KernelDefinitionId> m_Definitions;
-    ModifierFunction m_Function;
+    ConstraintFunction m_OnIntegers;
+    GenericConstraintFunction m_OnValues;
 };
 
 class KernelDefinition
 {
 public:
-    explicit KernelDefinition(const KernelDefinitionId id, const std::string& name, const std::string& source,
-        const DimensionVector& globalSize, const DimensionVector& localSize, const std::vector<std::string>& typeNames,
-        const std::string& sourceDirectory);
+    KernelDefinition(const KernelDefinitionId id, const std::string& name, const std::vector<std::string>& typeNames,
+        const std::string& source, const std::string& sourceDirectory, const DimensionVector& globalSize, const DimensionVector& localSize);
 
-    void SetArguments(const std::vector<KernelArgument*> arguments);
+    /** Binds arguments in kernel-signature order; an argument may appear only once. */
+    void SetArguments(const std::vector<KernelArgument*>& arguments);
 
     KernelDefinitionId GetId() const { return m_Id; }
     const std::string& GetName() const { return m_Name; }
+    const std::string& GetTemplatedName() const { return m_Templated; }
     const std::string& GetSource() const { return m_Source; }
-    const std::string& GetTemplatedName() const { return m_TemplatedName; }
-    const std::vector<std::string>& GetTypeNames() const { return m_TypeNames; }
-    const DimensionVector& GetGlobalSize() const { return m_GlobalSize; }
-    const DimensionVector& GetLocalSize() const { return m_LocalSize; }
+    /** Directory of the source file ("" for sources given as text); it is on the include path. */
+    const std::string& GetSourceDirectory() const { return m_Directory; }
+    const DimensionVector& GetGlobalSize() const { return m_Global; }
+    const DimensionVector& GetLocalSize() const { return m_Local; }
     const std::vector<KernelArgument*>& GetArguments() const { return m_Arguments; }
-    std::vector<KernelArgument*> GetVectorArguments() const;
+    std::vector<KernelArgument*> GetVectorArguments() const { return VectorArguments(m_Arguments); }
     bool HasArgument(const ArgumentId& id) const;
-    /** Directory of the file the source came from ("" for sources given as strings); added to the include path. */
-    const std::string& GetSourceDirectory() const { return m_SourceDirectory; }
-    bool IsTemplated() const { return !m_TypeNames.empty(); }
 
-    static std::string CreateTemplatedName(const std::string& name, const std::vector<std::string>& typeNames);
+    /** "name<T1, T2>" for template arguments T1, T2; the plain name without any. */
+    static std::string TemplatedName(const std::string& name, const std::vector<std::string>& typeNames);
 
 private:
     KernelDefinitionId m_Id;
     std::string m_Name;
+    std::string m_Templated;
     std::string m_Source;
-    std::string m_TemplatedName;
-    std::vector<std::string> m_TypeNames;
-    DimensionVector m_GlobalSize;
-    DimensionVector m_LocalSize;
+    std::string m_Directory;
+    DimensionVector m_Global;
+    DimensionVector m_Local;
     std::vector<KernelArgument*> m_Arguments;
-    std::string m_SourceDirectory;
 };
 
 class Kernel
 {
 public:
-    explicit Kernel(const KernelId id, const std::string& name, const std::vector<const KernelDefinition*>& definitions);
+    Kernel(const KernelId id, const std::string& name, const std::vector<const KernelDefinition*>& definitions);
 
     void AddParameter(const KernelParameter& parameter);
+    /** All of a constraint's parameters must exist and share one group. */
     void AddConstraint(const KernelConstraint& constraint);
-    void AddThreadModifier(const ModifierType type, const ModifierDimension dimension, const ThreadModifier& modifier);
-    void SetLauncher(KernelLauncher launcher);
+    void AddThreadModifier(const ModifierType type, const ModifierDimension dimension, const std::vector<std::string>& parameters,
+        const std::vector<KernelDefinitionId>& definitions, ModifierFunction function);
+    void SetLauncher(KernelLauncher launcher) { m_Launcher = launcher; }
     void SetProfiledDefinitions(const std::vector<const KernelDefinition*>& definitions);
 
     KernelId GetId() const { return m_Id; }
     const std::string& GetName() const { return m_Name; }
-    const KernelDefinition& GetPrimaryDefinition() const;
+    const KernelDefinition& GetPrimaryDefinition() const { return *m_Definitions.front(); }
     const KernelDefinition& GetDefinition(const KernelDefinitionId id) const;
     const std::vector<const KernelDefinition*>& GetDefinitions() const { return m_Definitions; }
-    const std::vector<const KernelDefinition*>& GetProfiledDefinitions() const { return m_ProfiledDefinitions; }
+    const std::vector<const KernelDefinition*>& GetProfiledDefinitions() const { return m_Profiled; }
     const std::vector<KernelParameter>& GetParameters() const { return m_Parameters; }
     const KernelParameter& GetParameter(const std::string& name) const;
     const std::vector<KernelConstraint>& GetConstraints() const { return m_Constraints; }
+    /** The vector arguments of all definitions, each once. */
     std::vector<KernelArgument*> GetVectorArguments() const;
     bool HasDefinition(const KernelDefinitionId id) const;
-    bool HasParameter(const std::string& name) const;
+    bool HasParameter(const std::string& name) const { return FindParameter(name) != nullptr; }
     bool HasLauncher() const { return static_cast<bool>(m_Launcher); }
     KernelLauncher GetLauncher() const { return m_Launcher; }
     bool IsComposite() const { return m_Definitions.size() > 1; }
 
+    /** A full configuration: the given values, and each other parameter's first value. */
     KernelConfiguration CreateConfiguration(const ParameterInput& parameters) const;
-    DimensionVector GetModifiedSize(const KernelDefinitionId id, const ModifierType type, const std::vector<ParameterPair>& pairs) const;
-    DimensionVector GetModifiedSize(const KernelDefinitionId id, const DimensionVector& originalSize, const ModifierType type,
+    /** `size` of definition `id` after the modifiers of `type` under the configuration `pairs`. */
+    DimensionVector GetModifiedSize(const KernelDefinitionId id, const DimensionVector& size, const ModifierType type,
         const std::vector<ParameterPair>& pairs) const;
 
 private:
+    struct Modifier
+    {
+        ModifierType type;
+        ModifierDimension dimension;
+        std::vector<std::string> parameters;
+        std::vector<KernelDefinitionId> definitions;
+        ModifierFunction function;
+    };
+
     KernelId m_Id;
     std::string m_Name;
     std::vector<const KernelDefinition*> m_Definitions;
-    std::vector<const KernelDefinition*> m_ProfiledDefinitions;
+    std::vector<const KernelDefinition*> m_Profiled;
     std::vector<KernelParameter> m_Parameters;
     std::vector<KernelConstraint> m_Constraints;
-    std::map<ModifierType, std::map<ModifierDimension, std::vector<ThreadModifier>>> m_Modifiers;
+    std::vector<Modifier> m_Modifiers;
     KernelLauncher m_Launcher;
+
+    const KernelParameter* FindParameter(const std::string& name) const;
 };
 
-class KernelManager
+class KernelRegistry
 {
 public:
-    explicit KernelManager(KernelArgumentManager& argumentManager);
+    explicit KernelRegistry(ArgumentRegistry& arguments);
 
-    KernelDefinitionId AddKernelDefinition(const std::string& name, const std::string& source, const DimensionVector& globalSize,
+    KernelDefinitionId AddDefinition(const std::string& name, const std::string& source, const DimensionVector& globalSize,
         const DimensionVector& localSize, const std::vector<std::string>& typeNames, const std::string& sourceDirectory = "");
-    KernelDefinitionId AddKernelDefinitionFromFile(const std::string& name, const std::string& filePath,
-        const DimensionVector& globalSize, const DimensionVector& localSize, const std::vector<std::string>& typeNames);
-    KernelDefinitionId GetDefinitionId(const std::string& name, const std::vector<std::string>& typeNames) const;
-    void RemoveKernelDefinition(const KernelDefinitionId id);
-    void SetArguments(const KernelDefinitionId id, const std::vector<ArgumentId>& argumentIds);
+    /** Reads the source from `filePath`; its directory goes on the include path. */
+    KernelDefinitionId AddDefinitionFromFile(const std::string& name, const std::string& filePath, const DimensionVector& globalSize,
+        const DimensionVector& localSize, const std::vector<std::string>& typeNames);
+    KernelDefinitionId FindDefinition(const std::string& name, const std::vector<std::string>& typeNames) const;
+    void RemoveDefinition(const KernelDefinitionId id);
+    void Bind(const KernelDefinitionId id, const std::vector<ArgumentId>& arguments);
 
-    KernelId CreateKernel(const std::string& name, const std::vector<KernelDefinitionId>& definitionIds);
+    KernelId AddKernel(const std::string& name, const std::vector<KernelDefinitionId>& definitions);
     void RemoveKernel(const KernelId id);
-    void SetLauncher(const KernelId id, KernelLauncher launcher);
-    void AddParameter(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values, const std::string& group);
-    void AddConstraint(const KernelId id, const std::vector<std::string>& parameters, ConstraintFunction function);
-    void AddGenericConstraint(const KernelId id, const std::vector<std::string>& parameters, GenericConstraintFunction function);
-    void AddThreadModifier(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds, const ModifierType type,
-        const ModifierDimension dimension, const std::vector<std::string>& parameters, ModifierFunction function);
-    void SetProfiledDefinitions(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds);
+    std::vector<const KernelDefinition*> Definitions(const std::vector<KernelDefinitionId>& ids) const;
 
     Kernel& GetKernel(const KernelId id);
     const Kernel& GetKernel(const KernelId id) const;
     KernelDefinition& GetDefinition(const KernelDefinitionId id);
     const KernelDefinition& GetDefinition(const KernelDefinitionId id) const;
-    bool HasKernel(const KernelId id) const;
-    bool HasDefinition(const KernelDefinitionId id) const;
-    bool IsArgumentUsed(const ArgumentId& id) const;
+    bool HasKernel(const KernelId id) const { return m_Kernels.count(id) != 0; }
+    bool HasDefinition(const KernelDefinitionId id) const { return m_Definitions.count(id) != 0; }
+    /** True when a definition still binds the argument. */
+    bool UsesArgument(const ArgumentId& id) const;
 
 private:
-    KernelArgumentManager& m_ArgumentManager;
+    ArgumentRegistry& m_Arguments;
     std::map<KernelDefinitionId, std::unique_ptr<KernelDefinition>> m_Definitions;
     std::map<KernelId, std::unique_ptr<Kernel>> m_Kernels;
-    KernelDefinitionId m_NextDefinitionId;
-    KernelId m_NextKernelId;
+    KernelDefinitionId m_NextDefinition;
+    KernelId m_NextKernel;
 };
 
-std::string LoadFileToString(const std::string& filePath);
+/** The whole content of a file; throws if it cannot be read. */
+std::string ReadTextFile(const std::string& path);
 
 } // namespace ktt

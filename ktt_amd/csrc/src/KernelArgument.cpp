@@ -2,7 +2,6 @@
 
 #include <KttException.h>
 
-#include "Common.h"
 #include "KernelArgument.h"
 
 namespace ktt
@@ -14,259 +13,174 @@ KernelArgument::KernelArgument(const ArgumentId& id, const size_t elementSize, c
     m_Id(id),
     m_ElementSize(elementSize),
     m_DataType(dataType),
-    m_MemoryLocation(memoryLocation),
-    m_AccessType(accessType),
-    m_MemoryType(memoryType),
-    m_ManagementType(managementType),
+    m_Location(memoryLocation),
+    m_Access(accessType),
+    m_Kind(memoryType),
+    m_Management(managementType),
     m_Ownership(ArgumentOwnership::Copy),
-    m_ReferencedData(nullptr),
-    m_UserBuffer(nullptr),
-    m_DataSize(0)
+    m_Host(nullptr),
+    m_Device(nullptr),
+    m_Bytes(0)
 {
     if (elementSize == 0)
     {
-        throw KttException("Kernel argument cannot have element size equal to zero");
+        throw KttException("Argument " + id + ": the element size must be positive");
+    }
+    if (memoryType == ArgumentMemoryType::Vector && memoryLocation == ArgumentMemoryLocation::Undefined)
+    {
+        throw KttException("Argument " + id + ": a vector needs a memory location");
     }
 }
 
 void KernelArgument::SetOwnedData(const void* data, const size_t dataSize)
 {
-    // Same preconditions as Source/KernelArgument/KernelArgument.cpp:67-88.
-    if (dataSize == 0 && m_MemoryType != ArgumentMemoryType::Local)
+    const bool local = m_Kind == ArgumentMemoryType::Local;
+    if (!local && (dataSize == 0 || data == nullptr))
     {
-        throw KttException("Kernel argument cannot be initialized with number of elements equal to zero");
+        throw KttException("Argument " + m_Id + ": data must be non-null and non-empty");
     }
-    if (data == nullptr && m_MemoryType != ArgumentMemoryType::Local)
-    {
-        throw KttException("Kernel argument cannot be initialized with null data");
-    }
-
     m_Ownership = ArgumentOwnership::Copy;
-    m_DataSize = dataSize;
-    m_ReferencedData = nullptr;
-    m_OwnedData.clear();
-
-    if (data != nullptr)
-    {
-        m_OwnedData.resize(dataSize);
-        std::memcpy(m_OwnedData.data(), data, dataSize);
-    }
+    m_Host = nullptr;
+    m_Device = nullptr;
+    m_Bytes = dataSize;
+    m_Copy.assign(static_cast<const uint8_t*>(data), data != nullptr ? static_cast<const uint8_t*>(data) + dataSize : nullptr);
 }
 
 void KernelArgument::SetReferencedData(void* data, const size_t dataSize)
 {
-    if (dataSize == 0)
+    if (dataSize == 0 || data == nullptr)
     {
-        throw KttException("Kernel argument cannot be initialized with number of elements equal to zero");
+        throw KttException("Argument " + m_Id + ": referenced data must be non-null and non-empty");
     }
-    if (data == nullptr)
-    {
-        throw KttException("Kernel argument cannot be initialized with null data");
-    }
-
     m_Ownership = ArgumentOwnership::Reference;
-    m_ReferencedData = data;
-    m_DataSize = dataSize;
-    m_OwnedData.clear();
+    m_Copy.clear();
+    m_Device = nullptr;
+    m_Host = data;
+    m_Bytes = dataSize;
 }
 
 void KernelArgument::SetUserBuffer(ComputeBuffer buffer, const size_t dataSize)
 {
     if (dataSize == 0)
     {
-        throw KttException("Kernel argument cannot be initialized with number of elements equal to zero");
+        throw KttException("Argument " + m_Id + ": a device buffer must have a positive size");
     }
-
     m_Ownership = ArgumentOwnership::User;
-    m_UserBuffer = buffer;
-    m_DataSize = dataSize;
-    m_OwnedData.clear();
-    m_ReferencedData = nullptr;
-}
-
-void KernelArgument::UpdateScalarData(const void* data)
-{
-    if (m_MemoryType != ArgumentMemoryType::Scalar)
-    {
-        throw KttException("Kernel argument with id " + m_Id + " is not a scalar");
-    }
-    std::memcpy(m_OwnedData.data(), data, m_DataSize);
+    m_Copy.clear();
+    m_Host = nullptr;
+    m_Device = buffer;
+    m_Bytes = dataSize;
 }
 
 void KernelArgument::SetLocalSize(const size_t dataSize)
 {
-    if (m_MemoryType != ArgumentMemoryType::Local)
+    if (m_Kind != ArgumentMemoryType::Local)
     {
-        throw KttException("Kernel argument with id " + m_Id + " is not a local memory argument");
+        throw KttException("Argument " + m_Id + " does not describe local memory");
     }
-    m_DataSize = dataSize;
+    m_Bytes = dataSize;
 }
 
 const void* KernelArgument::GetData() const
 {
-    switch (m_Ownership)
+    if (m_Ownership == ArgumentOwnership::User)
     {
-    case ArgumentOwnership::Copy:
-        return m_OwnedData.empty() ? nullptr : m_OwnedData.data();
-    case ArgumentOwnership::Reference:
-        return m_ReferencedData;
-    default:
-        throw KttException("Host data of kernel argument with id " + m_Id + " is not accessible, it is a user buffer");
+        throw KttException("Argument " + m_Id + " is a device buffer and has no host data");
     }
+    if (m_Ownership == ArgumentOwnership::Reference)
+    {
+        return m_Host;
+    }
+    return m_Copy.empty() ? nullptr : m_Copy.data();
 }
 
 void* KernelArgument::GetData()
 {
-    return const_cast<void*>(static_cast<const KernelArgument&>(*this).GetData());
+    return const_cast<void*>(static_cast<const KernelArgument*>(this)->GetData());
 }
 
-std::vector<KernelArgument*> KernelArgument::GetArgumentsWithMemoryType(const std::vector<KernelArgument*>& arguments,
-    const ArgumentMemoryType type)
+std::vector<KernelArgument*> VectorArguments(const std::vector<KernelArgument*>& arguments)
 {
-    std::vector<KernelArgument*> result;
-    for (auto* argument : arguments)
+    std::vector<KernelArgument*> vectors;
+    for (KernelArgument* argument : arguments)
     {
-        if (argument->GetMemoryType() == type)
+        if (argument->GetMemoryType() == ArgumentMemoryType::Vector)
         {
-            result.push_back(argument);
+            vectors.push_back(argument);
         }
     }
-    return result;
+    return vectors;
 }
 
-KernelArgumentManager::KernelArgumentManager() :
-    m_NextId(0)
-{}
+ArgumentRegistry::ArgumentRegistry() : m_Counter(0) {}
 
-ArgumentId KernelArgumentManager::ResolveId(const ArgumentId& customId)
+ArgumentId ArgumentRegistry::Add(const ArgumentSpec& spec, const ArgumentSource source, void* data, const size_t bytes,
+    const ArgumentId& name)
 {
-    if (!customId.empty())
+    ArgumentId id = name;
+    if (id.empty())
     {
-        if (HasArgument(customId))
+        do
         {
-            throw KttException("Kernel argument with id " + customId + " already exists");
+            id = std::to_string(m_Counter++);
         }
-        return customId;
+        while (Contains(id));
+    }
+    else if (Contains(id))
+    {
+        throw KttException("An argument named " + id + " exists already");
     }
 
-    ArgumentId id;
-    do
+    auto argument = std::make_unique<KernelArgument>(id, spec.elementSize, spec.dataType, spec.location, spec.access, spec.kind,
+        spec.management);
+    switch (source)
     {
-        id = std::to_string(m_NextId++);
+    case ArgumentSource::Copy:
+        argument->SetOwnedData(data, bytes);
+        break;
+    case ArgumentSource::Reference:
+        argument->SetReferencedData(data, bytes);
+        break;
+    case ArgumentSource::DeviceBuffer:
+        argument->SetUserBuffer(data, bytes);
+        break;
     }
-    while (HasArgument(id));
+    m_Arguments.emplace(id, std::move(argument));
     return id;
 }
 
-KernelArgument& KernelArgumentManager::AddArgument(const ArgumentId& id, const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation location, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-    const ArgumentManagementType managementType, [[maybe_unused]] const size_t dataSize)
-{
-    if (memoryType == ArgumentMemoryType::Vector && location == ArgumentMemoryLocation::Undefined)
-    {
-        throw KttException("Vector kernel arguments must have properly defined memory location");
-    }
-    auto argument = std::make_unique<KernelArgument>(id, elementSize, dataType, location, accessType, memoryType, managementType);
-    auto& reference = *argument;
-    m_Arguments[id] = std::move(argument);
-    return reference;
-}
-
-ArgumentId KernelArgumentManager::AddArgumentWithReferencedData(const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation location, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-    const ArgumentManagementType managementType, void* data, const size_t dataSize, const ArgumentId& customId)
-{
-    const ArgumentId id = ResolveId(customId);
-    auto& argument = AddArgument(id, elementSize, dataType, location, accessType, memoryType, managementType, dataSize);
-    try
-    {
-        argument.SetReferencedData(data, dataSize);
-    }
-    catch (const KttException&)
-    {
-        m_Arguments.erase(id);
-        throw;
-    }
-    return id;
-}
-
-ArgumentId KernelArgumentManager::AddArgumentWithOwnedData(const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation location, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-    const ArgumentManagementType managementType, const void* data, const size_t dataSize, const ArgumentId& customId)
-{
-    const ArgumentId id = ResolveId(customId);
-    auto& argument = AddArgument(id, elementSize, dataType, location, accessType, memoryType, managementType, dataSize);
-    try
-    {
-        argument.SetOwnedData(data, dataSize);
-    }
-    catch (const KttException&)
-    {
-        m_Arguments.erase(id);
-        throw;
-    }
-    return id;
-}
-
-ArgumentId KernelArgumentManager::AddUserArgument(ComputeBuffer buffer, const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation location, const ArgumentAccessType accessType, const size_t dataSize, const ArgumentId& customId)
-{
-    const ArgumentId id = ResolveId(customId);
-    auto& argument = AddArgument(id, elementSize, dataType, location, accessType, ArgumentMemoryType::Vector,
-        ArgumentManagementType::User, dataSize);
-    try
-    {
-        argument.SetUserBuffer(buffer, dataSize);
-    }
-    catch (const KttException&)
-    {
-        m_Arguments.erase(id);
-        throw;
-    }
-    return id;
-}
-
-void KernelArgumentManager::RemoveArgument(const ArgumentId& id)
+void ArgumentRegistry::Remove(const ArgumentId& id)
 {
     if (m_Arguments.erase(id) == 0)
     {
-        throw KttException("Attempting to remove argument with invalid id: " + id);
+        throw KttException("No argument named " + id + " to remove");
     }
 }
 
-KernelArgument& KernelArgumentManager::GetArgument(const ArgumentId& id)
+KernelArgument& ArgumentRegistry::Get(const ArgumentId& id)
 {
-    auto iterator = m_Arguments.find(id);
-    if (iterator == m_Arguments.end())
+    return const_cast<KernelArgument&>(static_cast<const ArgumentRegistry*>(this)->Get(id));
+}
+
+const KernelArgument& ArgumentRegistry::Get(const ArgumentId& id) const
+{
+    const auto found = m_Arguments.find(id);
+    if (found == m_Arguments.end())
     {
-        throw KttException("Attempting to retrieve argument with invalid id: " + id);
+        throw KttException("No argument named " + id);
     }
-    return *iterator->second;
+    return *found->second;
 }
 
-const KernelArgument& KernelArgumentManager::GetArgument(const ArgumentId& id) const
+std::vector<KernelArgument*> ArgumentRegistry::Resolve(const std::vector<ArgumentId>& ids)
 {
-    auto iterator = m_Arguments.find(id);
-    if (iterator == m_Arguments.end())
+    std::vector<KernelArgument*> arguments;
+    arguments.reserve(ids.size());
+    for (const ArgumentId& id : ids)
     {
-        throw KttException("Attempting to retrieve argument with invalid id: " + id);
+        arguments.push_back(&Get(id));
     }
-    return *iterator->second;
-}
-
-std::vector<KernelArgument*> KernelArgumentManager::GetArguments(const std::vector<ArgumentId>& ids)
-{
-    std::vector<KernelArgument*> result;
-    for (const auto& id : ids)
-    {
-        result.push_back(&GetArgument(id));
-    }
-    return result;
-}
-
-bool KernelArgumentManager::HasArgument(const ArgumentId& id) const
-{
-    return ContainsKey(m_Arguments, id);
+    return arguments;
 }
 
 } // namespace ktt

@@ -1,21 +1,71 @@
-// ktt::Tuner facade. Every method forwards to TunerCore and, like Source/Tuner.cpp of the
-// reference, catches KttException, logs it at error level and returns the Invalid* value
-// (or an empty result) instead of throwing. Constructors throw, as in the reference.
+// ktt::Tuner. Each public method forwards to TunerCore inside Guarded(): a KttException is
+// logged at error level and the method returns its "invalid" value (InvalidKernelId, an
+// empty result, ...) instead of throwing, which is the reference's contract for this class
+// (Source/Tuner.cpp:41-52). The constructors do throw.
+#include <fstream>
+
 #include <Ktt.h>
 
 #include "Common.h"
-#include "HipEngine.h"
 #include "TunerCore.h"
 
 namespace ktt
 {
 
-#define KTT_TRY try {
-#define KTT_CATCH_RETURN(value) } catch (const KttException& exception) { TunerCore::Log(LoggingLevel::Error, exception.what()); return value; }
-#define KTT_CATCH } catch (const KttException& exception) { TunerCore::Log(LoggingLevel::Error, exception.what()); }
+namespace
+{
+
+template <typename Result, typename Body>
+Result Guarded(Result fallback, Body body)
+{
+    try
+    {
+        return body();
+    }
+    catch (const KttException& failure)
+    {
+        Logger::LogError(failure.what());
+        return fallback;
+    }
+}
+
+template <typename Body>
+void Guarded(Body body)
+{
+    try
+    {
+        body();
+    }
+    catch (const KttException& failure)
+    {
+        Logger::LogError(failure.what());
+    }
+}
+
+ModifierFunction ActionFunction(const ModifierAction action)
+{
+    switch (action)
+    {
+    case ModifierAction::Add:
+        return [](const uint64_t size, const std::vector<uint64_t>& v) { return size + v[0]; };
+    case ModifierAction::Subtract:
+        return [](const uint64_t size, const std::vector<uint64_t>& v) { return size - v[0]; };
+    case ModifierAction::Multiply:
+        return [](const uint64_t size, const std::vector<uint64_t>& v) { return size * v[0]; };
+    case ModifierAction::Divide:
+        return [](const uint64_t size, const std::vector<uint64_t>& v) { return size / v[0]; };
+    case ModifierAction::DivideCeil:
+        return [](const uint64_t size, const std::vector<uint64_t>& v) { return (size + v[0] - 1) / v[0]; };
+    }
+    throw KttException("Unknown thread modifier action");
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- construction
 
 Tuner::Tuner(const PlatformIndex platform, const DeviceIndex device, const ComputeApi api) :
-    m_Tuner(std::make_unique<TunerCore>(platform, device, api, 1))
+    Tuner(platform, device, api, 1)
 {}
 
 Tuner::Tuner(const PlatformIndex platform, const DeviceIndex device, const ComputeApi api, const uint32_t computeQueueCount) :
@@ -24,8 +74,8 @@ Tuner::Tuner(const PlatformIndex platform, const DeviceIndex device, const Compu
 
 Tuner::Tuner(const ComputeApi api, const ComputeApiInitializer& initializer)
 {
-    std::vector<QueueId> ids;
-    m_Tuner = std::make_unique<TunerCore>(api, initializer, ids);
+    std::vector<QueueId> unused;
+    m_Tuner = std::make_unique<TunerCore>(api, initializer, unused);
 }
 
 Tuner::Tuner(const ComputeApi api, const ComputeApiInitializer& initializer, std::vector<QueueId>& assignedQueueIds) :
@@ -34,181 +84,177 @@ Tuner::Tuner(const ComputeApi api, const ComputeApiInitializer& initializer, std
 
 Tuner::~Tuner() = default;
 
+// ---------------------------------------------------------------- kernels
+
 KernelDefinitionId Tuner::AddKernelDefinition(const std::string& name, const std::string& source, const DimensionVector& globalSize,
     const DimensionVector& localSize, const std::vector<std::string>& typeNames)
 {
-    KTT_TRY
-    return m_Tuner->GetKernelManager().AddKernelDefinition(name, source, globalSize, localSize, typeNames);
-    KTT_CATCH_RETURN(InvalidKernelDefinitionId)
+    return Guarded(InvalidKernelDefinitionId,
+        [&]() { return m_Tuner->GetKernels().AddDefinition(name, source, globalSize, localSize, typeNames); });
 }
 
 KernelDefinitionId Tuner::AddKernelDefinitionFromFile(const std::string& name, const std::string& filePath,
     const DimensionVector& globalSize, const DimensionVector& localSize, const std::vector<std::string>& typeNames)
 {
-    KTT_TRY
-    return m_Tuner->GetKernelManager().AddKernelDefinitionFromFile(name, filePath, globalSize, localSize, typeNames);
-    KTT_CATCH_RETURN(InvalidKernelDefinitionId)
+    return Guarded(InvalidKernelDefinitionId,
+        [&]() { return m_Tuner->GetKernels().AddDefinitionFromFile(name, filePath, globalSize, localSize, typeNames); });
 }
 
 KernelDefinitionId Tuner::GetKernelDefinitionId(const std::string& name, const std::vector<std::string>& typeNames) const
 {
-    KTT_TRY
-    return m_Tuner->GetKernelManager().GetDefinitionId(name, typeNames);
-    KTT_CATCH_RETURN(InvalidKernelDefinitionId)
+    return Guarded(InvalidKernelDefinitionId, [&]() { return m_Tuner->GetKernels().FindDefinition(name, typeNames); });
 }
 
 void Tuner::RemoveKernelDefinition(const KernelDefinitionId id)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().RemoveKernelDefinition(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetKernels().RemoveDefinition(id); });
 }
 
 void Tuner::SetArguments(const KernelDefinitionId id, const std::vector<ArgumentId>& argumentIds)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().SetArguments(id, argumentIds);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetKernels().Bind(id, argumentIds); });
 }
 
 KernelId Tuner::CreateSimpleKernel(const std::string& name, const KernelDefinitionId definitionId)
 {
-    KTT_TRY
-    return m_Tuner->GetKernelManager().CreateKernel(name, {definitionId});
-    KTT_CATCH_RETURN(InvalidKernelId)
+    return Guarded(InvalidKernelId, [&]() { return m_Tuner->GetKernels().AddKernel(name, {definitionId}); });
 }
 
 KernelId Tuner::CreateCompositeKernel(const std::string& name, const std::vector<KernelDefinitionId>& definitionIds,
     KernelLauncher launcher)
 {
-    KTT_TRY
-    const KernelId id = m_Tuner->GetKernelManager().CreateKernel(name, definitionIds);
-    if (launcher)
+    return Guarded(InvalidKernelId, [&]()
     {
-        m_Tuner->GetKernelManager().SetLauncher(id, launcher);
-    }
-    return id;
-    KTT_CATCH_RETURN(InvalidKernelId)
+        const KernelId id = m_Tuner->GetKernels().AddKernel(name, definitionIds);
+        if (launcher)
+        {
+            m_Tuner->GetKernels().GetKernel(id).SetLauncher(launcher);
+        }
+        return id;
+    });
 }
 
 void Tuner::RemoveKernel(const KernelId id)
 {
-    KTT_TRY
-    m_Tuner->RemoveKernel(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->RemoveKernel(id); });
 }
 
 void Tuner::SetLauncher(const KernelId id, KernelLauncher launcher)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().SetLauncher(id, launcher);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetKernels().GetKernel(id).SetLauncher(launcher); });
+}
+
+void Tuner::AddParameterValues(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values,
+    const std::string& group)
+{
+    Guarded([&]() { m_Tuner->GetKernels().GetKernel(id).AddParameter(KernelParameter(name, values, group)); });
 }
 
 void Tuner::AddThreadModifier(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds, const ModifierType type,
     const ModifierDimension dimension, const std::vector<std::string>& parameters, ModifierFunction function)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().AddThreadModifier(id, definitionIds, type, dimension, parameters, function);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetKernels().GetKernel(id).AddThreadModifier(type, dimension, parameters, definitionIds, function); });
 }
 
 void Tuner::AddThreadModifier(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds, const ModifierType type,
     const ModifierDimension dimension, const std::string& parameter, const ModifierAction action)
 {
-    // Actions as in Source/Tuner.cpp:194-233.
-    ModifierFunction function;
-    switch (action)
+    Guarded([&]()
     {
-    case ModifierAction::Add:
-        function = [](const uint64_t size, const std::vector<uint64_t>& values) { return size + values[0]; };
-        break;
-    case ModifierAction::Subtract:
-        function = [](const uint64_t size, const std::vector<uint64_t>& values) { return size - values[0]; };
-        break;
-    case ModifierAction::Multiply:
-        function = [](const uint64_t size, const std::vector<uint64_t>& values) { return size * values[0]; };
-        break;
-    case ModifierAction::Divide:
-        function = [](const uint64_t size, const std::vector<uint64_t>& values) { return size / values[0]; };
-        break;
-    case ModifierAction::DivideCeil:
-        function = [](const uint64_t size, const std::vector<uint64_t>& values) { return (size + values[0] - 1) / values[0]; };
-        break;
-    }
-    AddThreadModifier(id, definitionIds, type, dimension, std::vector<std::string>{parameter}, function);
+        m_Tuner->GetKernels().GetKernel(id).AddThreadModifier(type, dimension, {parameter}, definitionIds, ActionFunction(action));
+    });
 }
 
 void Tuner::AddConstraint(const KernelId id, const std::vector<std::string>& parameters, ConstraintFunction function)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().AddConstraint(id, parameters, function);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetKernels().GetKernel(id).AddConstraint(KernelConstraint(parameters, function)); });
 }
 
 void Tuner::AddGenericConstraint(const KernelId id, const std::vector<std::string>& parameters, GenericConstraintFunction function)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().AddGenericConstraint(id, parameters, function);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetKernels().GetKernel(id).AddConstraint(KernelConstraint(parameters, function)); });
 }
 
 void Tuner::SetProfiledDefinitions(const KernelId id, const std::vector<KernelDefinitionId>& definitionIds)
 {
-    KTT_TRY
-    m_Tuner->GetKernelManager().SetProfiledDefinitions(id, definitionIds);
-    KTT_CATCH
+    Guarded([&]()
+    {
+        KernelRegistry& kernels = m_Tuner->GetKernels();
+        kernels.GetKernel(id).SetProfiledDefinitions(kernels.Definitions(definitionIds));
+    });
+}
+
+// ---------------------------------------------------------------- arguments
+
+ArgumentId Tuner::RegisterArgument(const size_t elementSize, const ArgumentDataType dataType, const ArgumentMemoryLocation location,
+    const ArgumentAccessType access, const ArgumentMemoryType kind, const ArgumentManagementType management, const DataSource source,
+    void* data, const size_t bytes, const ArgumentId& name)
+{
+    return Guarded(InvalidArgumentId, [&]()
+    {
+        ArgumentSpec spec;
+        spec.elementSize = elementSize;
+        spec.dataType = dataType;
+        spec.location = location;
+        spec.access = access;
+        spec.kind = kind;
+        spec.management = management;
+        const ArgumentSource from = source == DataSource::Reference ? ArgumentSource::Reference
+            : source == DataSource::DeviceBuffer ? ArgumentSource::DeviceBuffer : ArgumentSource::Copy;
+        return m_Tuner->GetArguments().Add(spec, from, data, bytes, name);
+    });
 }
 
 ArgumentId Tuner::AddArgumentVector(ComputeBuffer buffer, const size_t bufferSize, const size_t elementSize,
     const ArgumentAccessType accessType, const ArgumentMemoryLocation memoryLocation, const ArgumentId& customId)
 {
-    return AddUserArgument(buffer, elementSize, ArgumentDataType::Custom, memoryLocation, accessType, bufferSize, customId);
+    return RegisterArgument(elementSize, ArgumentDataType::Custom, memoryLocation, accessType, ArgumentMemoryType::Vector,
+        ArgumentManagementType::User, DataSource::DeviceBuffer, buffer, bufferSize, customId);
 }
 
 ArgumentId Tuner::AddArgumentVectorFromFile(const std::string& filePath, const ArgumentDataType dataType, const size_t elementSize,
     const ArgumentAccessType accessType, const ArgumentMemoryLocation memoryLocation, const ArgumentManagementType managementType,
     const ArgumentId& customId)
 {
-    KTT_TRY
-    const std::string data = LoadFileToString(filePath);
-    return m_Tuner->GetArgumentManager().AddArgumentWithOwnedData(elementSize, dataType, memoryLocation, accessType,
-        ArgumentMemoryType::Vector, managementType, data.data(), data.size(), customId);
-    KTT_CATCH_RETURN(InvalidArgumentId)
+    return Guarded(InvalidArgumentId, [&]()
+    {
+        std::string bytes = ReadTextFile(filePath);
+        return RegisterArgument(elementSize, dataType, memoryLocation, accessType, ArgumentMemoryType::Vector, managementType,
+            DataSource::Copy, bytes.data(), bytes.size(), customId);
+    });
 }
 
 void Tuner::SaveArgumentVector(const ArgumentId& id, const std::string& filePath)
 {
-    KTT_TRY
-    const auto& argument = m_Tuner->GetArgumentManager().GetArgument(id);
-    std::ofstream stream(filePath, std::ios::binary);
-    if (!stream.is_open())
+    Guarded([&]()
     {
-        throw KttException("Unable to open file: " + filePath);
-    }
-    stream.write(static_cast<const char*>(argument.GetData()), static_cast<std::streamsize>(argument.GetDataSize()));
-    KTT_CATCH
+        const KernelArgument& argument = m_Tuner->GetArguments().Get(id);
+        std::ofstream file(filePath, std::ios::binary);
+        if (!file)
+        {
+            throw KttException("Cannot write " + filePath);
+        }
+        file.write(static_cast<const char*>(argument.GetData()), static_cast<std::streamsize>(argument.GetDataSize()));
+    });
 }
 
 ArgumentId Tuner::AddArgumentScalar(const void* data, const size_t dataSize, const ArgumentId& customId)
 {
-    return AddArgumentWithOwnedData(dataSize, ArgumentDataType::Custom, ArgumentMemoryLocation::Undefined, ArgumentAccessType::ReadOnly,
-        ArgumentMemoryType::Scalar, ArgumentManagementType::Framework, data, dataSize, customId);
+    return RegisterArgument(dataSize, ArgumentDataType::Custom, ArgumentMemoryLocation::Undefined, ArgumentAccessType::ReadOnly,
+        ArgumentMemoryType::Scalar, ArgumentManagementType::Framework, DataSource::Copy, const_cast<void*>(data), dataSize, customId);
 }
 
 void Tuner::RemoveArgument(const ArgumentId& id)
 {
-    KTT_TRY
-    m_Tuner->RemoveArgument(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->RemoveArgument(id); });
 }
 
 void Tuner::SetReadOnlyArgumentCache(const bool flag)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().KeepReadOnlyBuffers(flag);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().KeepReadOnlyBuffers(flag); });
 }
+
+// ---------------------------------------------------------------- running and validation
 
 KernelResult Tuner::Run(const KernelId id, const KernelConfiguration& configuration, const std::vector<BufferOutputDescriptor>& output)
 {
@@ -218,77 +264,68 @@ KernelResult Tuner::Run(const KernelId id, const KernelConfiguration& configurat
 KernelResult Tuner::Run(const KernelId id, const KernelConfiguration& configuration, const KernelDimensions& dimensions,
     const std::vector<BufferOutputDescriptor>& output)
 {
-    KTT_TRY
-    const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetExecutor().Execute(kernel, configuration, dimensions, KernelRunMode::Running, output);
-    KTT_CATCH_RETURN(KernelResult())
+    return Guarded(KernelResult(), [&]()
+    {
+        return m_Tuner->GetExecutor().Execute(m_Tuner->GetKernels().GetKernel(id), configuration, dimensions, KernelRunMode::Running,
+            output);
+    });
 }
 
 void Tuner::SetProfiling(const bool flag)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().SetProfiling(flag);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().SetProfiling(flag); });
 }
 
 bool Tuner::GetProfiling()
 {
-    KTT_TRY
-    return m_Tuner->GetExecutor().IsProfiling();
-    KTT_CATCH_RETURN(false)
+    return Guarded(false, [&]() { return m_Tuner->GetExecutor().IsProfiling(); });
 }
 
 void Tuner::SetValidationMethod(const ValidationMethod method, const double toleranceThreshold)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().Checker().SetMethod(method, toleranceThreshold);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().Checker().SetMethod(method, toleranceThreshold); });
 }
 
 void Tuner::SetValidationMode(const ValidationMode mode)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().Checker().SetModes(mode);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().Checker().SetModes(mode); });
 }
 
 void Tuner::SetValidationRange(const ArgumentId& id, const size_t range)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().Checker().SetRange(m_Tuner->GetArgumentManager().GetArgument(id), range);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().Checker().SetRange(m_Tuner->GetArguments().Get(id), range); });
 }
 
 void Tuner::SetValueComparator(const ArgumentId& id, ValueComparator comparator)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().Checker().SetComparator(m_Tuner->GetArgumentManager().GetArgument(id), comparator);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().Checker().SetComparator(m_Tuner->GetArguments().Get(id), comparator); });
 }
 
 void Tuner::SetReferenceComputation(const ArgumentId& id, ReferenceComputation computation)
 {
-    KTT_TRY
-    m_Tuner->GetExecutor().Checker().SetComputation(m_Tuner->GetArgumentManager().GetArgument(id), computation);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetExecutor().Checker().SetComputation(m_Tuner->GetArguments().Get(id), computation); });
 }
 
 void Tuner::SetReferenceKernel(const ArgumentId& id, const KernelId referenceId, const KernelConfiguration& configuration,
     const KernelDimensions& dimensions)
 {
-    KTT_TRY
-    const auto& kernel = m_Tuner->GetKernelManager().GetKernel(referenceId);
-    m_Tuner->GetExecutor().Checker().SetKernel(m_Tuner->GetArgumentManager().GetArgument(id), kernel, configuration, dimensions);
-    KTT_CATCH
+    Guarded([&]()
+    {
+        m_Tuner->GetExecutor().Checker().SetKernel(m_Tuner->GetArguments().Get(id), m_Tuner->GetKernels().GetKernel(referenceId),
+            configuration, dimensions);
+    });
 }
 
 void Tuner::SetReferenceArgument(const ArgumentId& id, const ArgumentId& referenceId)
 {
-    KTT_TRY
-    const auto& argument = m_Tuner->GetArgumentManager().GetArgument(referenceId);
-    m_Tuner->GetExecutor().Checker().SetArgument(m_Tuner->GetArgumentManager().GetArgument(id), argument);
-    KTT_CATCH
+    Guarded([&]()
+    {
+        ArgumentRegistry& arguments = m_Tuner->GetArguments();
+        m_Tuner->GetExecutor().Checker().SetArgument(arguments.Get(id), arguments.Get(referenceId));
+    });
 }
+
+// ---------------------------------------------------------------- tuning
 
 std::vector<KernelResult> Tuner::Tune(const KernelId id, std::unique_ptr<StopCondition> stopCondition)
 {
@@ -297,10 +334,10 @@ std::vector<KernelResult> Tuner::Tune(const KernelId id, std::unique_ptr<StopCon
 
 std::vector<KernelResult> Tuner::Tune(const KernelId id, const KernelDimensions& dimensions, std::unique_ptr<StopCondition> stopCondition)
 {
-    KTT_TRY
-    const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetTuningLoop().Tune(kernel, dimensions, std::move(stopCondition));
-    KTT_CATCH_RETURN(std::vector<KernelResult>{})
+    return Guarded(std::vector<KernelResult>{}, [&]()
+    {
+        return m_Tuner->GetTuningLoop().Tune(m_Tuner->GetKernels().GetKernel(id), dimensions, std::move(stopCondition));
+    });
 }
 
 KernelResult Tuner::TuneIteration(const KernelId id, const std::vector<BufferOutputDescriptor>& output, const bool recomputeReference)
@@ -311,84 +348,74 @@ KernelResult Tuner::TuneIteration(const KernelId id, const std::vector<BufferOut
 KernelResult Tuner::TuneIteration(const KernelId id, const KernelDimensions& dimensions, const std::vector<BufferOutputDescriptor>& output,
     const bool recomputeReference)
 {
-    KTT_TRY
-    const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetTuningLoop().Step(kernel, dimensions, KernelRunMode::OnlineTuning, output, recomputeReference);
-    KTT_CATCH_RETURN(KernelResult())
+    return Guarded(KernelResult(), [&]()
+    {
+        return m_Tuner->GetTuningLoop().Step(m_Tuner->GetKernels().GetKernel(id), dimensions, KernelRunMode::OnlineTuning, output,
+            recomputeReference);
+    });
 }
 
 std::vector<KernelResult> Tuner::SimulateTuning(const KernelId id, const std::vector<KernelResult>& results,
     std::unique_ptr<StopCondition> stopCondition)
 {
-    KTT_TRY
-    const auto& kernel = m_Tuner->GetKernelManager().GetKernel(id);
-    return m_Tuner->GetTuningLoop().Replay(kernel, results, std::move(stopCondition));
-    KTT_CATCH_RETURN(std::vector<KernelResult>{})
+    return Guarded(std::vector<KernelResult>{}, [&]()
+    {
+        return m_Tuner->GetTuningLoop().Replay(m_Tuner->GetKernels().GetKernel(id), results, std::move(stopCondition));
+    });
 }
 
 void Tuner::SetSearcher(const KernelId id, std::unique_ptr<Searcher> searcher)
 {
-    KTT_TRY
-    if (!m_Tuner->GetKernelManager().HasKernel(id))
+    Guarded([&]()
     {
-        throw KttException("Attempting to set searcher for kernel with invalid id: " + std::to_string(id));
-    }
-    if (searcher == nullptr)
-    {
-        throw KttException("Searcher must be a valid object");
-    }
-    m_Tuner->GetTuningLoop().SetSearcher(id, std::move(searcher));
-    KTT_CATCH
+        if (!m_Tuner->GetKernels().HasKernel(id))
+        {
+            throw KttException("No kernel with id " + std::to_string(id) + " to set a searcher for");
+        }
+        if (searcher == nullptr)
+        {
+            throw KttException("The searcher must not be null");
+        }
+        m_Tuner->GetTuningLoop().SetSearcher(id, std::move(searcher));
+    });
 }
 
 void Tuner::InitializeConfigurationData(const KernelId id)
 {
-    KTT_TRY
-    m_Tuner->GetTuningLoop().Prepare(m_Tuner->GetKernelManager().GetKernel(id));
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetTuningLoop().Prepare(m_Tuner->GetKernels().GetKernel(id)); });
 }
 
 void Tuner::ClearConfigurationData(const KernelId id)
 {
-    KTT_TRY
-    m_Tuner->GetTuningLoop().Reset(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetTuningLoop().Reset(id); });
 }
 
 uint64_t Tuner::GetConfigurationsCount(const KernelId id) const
 {
-    KTT_TRY
-    return m_Tuner->GetTuningLoop().SpaceSize(id);
-    KTT_CATCH_RETURN(0)
+    return Guarded(uint64_t{0}, [&]() { return m_Tuner->GetTuningLoop().SpaceSize(id); });
 }
 
 KernelConfiguration Tuner::GetBestConfiguration(const KernelId id) const
 {
-    KTT_TRY
-    return m_Tuner->GetTuningLoop().Best(id);
-    KTT_CATCH_RETURN(KernelConfiguration())
+    return Guarded(KernelConfiguration(), [&]() { return m_Tuner->GetTuningLoop().Best(id); });
 }
 
 KernelConfiguration Tuner::CreateConfiguration(const KernelId id, const ParameterInput& parameters) const
 {
-    KTT_TRY
-    return m_Tuner->GetKernelManager().GetKernel(id).CreateConfiguration(parameters);
-    KTT_CATCH_RETURN(KernelConfiguration())
+    return Guarded(KernelConfiguration(), [&]() { return m_Tuner->GetKernels().GetKernel(id).CreateConfiguration(parameters); });
 }
 
 std::string Tuner::GetKernelSource(const KernelId id, const KernelConfiguration& configuration) const
 {
-    KTT_TRY
-    return m_Tuner->GetKernelSource(id, configuration);
-    KTT_CATCH_RETURN(std::string())
+    return Guarded(std::string(), [&]() { return m_Tuner->GetKernelSource(id, configuration); });
 }
 
 std::string Tuner::GetKernelDefinitionSource(const KernelDefinitionId id, const KernelConfiguration& configuration) const
 {
-    KTT_TRY
-    return m_Tuner->GetKernelDefinitionSource(id, configuration);
-    KTT_CATCH_RETURN(std::string())
+    return Guarded(std::string(), [&]() { return m_Tuner->GetKernelDefinitionSource(id, configuration); });
 }
+
+// ---------------------------------------------------------------- results
 
 void Tuner::SetTimeUnit(const TimeUnit unit)
 {
@@ -398,128 +425,98 @@ void Tuner::SetTimeUnit(const TimeUnit unit)
 void Tuner::SaveResults(const std::vector<KernelResult>& results, const std::string& filePath, const OutputFormat format,
     const UserData& data) const
 {
-    KTT_TRY
-    m_Tuner->SaveResults(results, filePath, format, data);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->SaveResults(results, filePath, format, data); });
 }
 
 std::vector<KernelResult> Tuner::LoadResults(const std::string& filePath, const OutputFormat format) const
 {
-    UserData data;
-    return LoadResults(filePath, format, data);
+    UserData ignored;
+    return LoadResults(filePath, format, ignored);
 }
 
 std::vector<KernelResult> Tuner::LoadResults(const std::string& filePath, const OutputFormat format, UserData& data) const
 {
-    KTT_TRY
-    return m_Tuner->LoadResults(filePath, format, data);
-    KTT_CATCH_RETURN(std::vector<KernelResult>{})
+    return Guarded(std::vector<KernelResult>{}, [&]() { return m_Tuner->LoadResults(filePath, format, data); });
 }
+
+// ---------------------------------------------------------------- queues and engine settings
 
 QueueId Tuner::AddComputeQueue(ComputeQueue queue)
 {
-    KTT_TRY
-    return m_Tuner->GetComputeEngine().AddComputeQueue(queue);
-    KTT_CATCH_RETURN(InvalidQueueId)
+    return Guarded(InvalidQueueId, [&]() { return m_Tuner->GetComputeEngine().AddComputeQueue(queue); });
 }
 
 void Tuner::RemoveComputeQueue(const QueueId id)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().RemoveComputeQueue(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().RemoveComputeQueue(id); });
 }
 
 void Tuner::WaitForComputeAction(const ComputeActionId id)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().WaitForComputeAction(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().WaitForComputeAction(id); });
 }
 
 void Tuner::WaitForTransferAction(const TransferActionId id)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().WaitForTransferAction(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().WaitForTransferAction(id); });
 }
 
 void Tuner::SynchronizeQueue(const QueueId id)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SynchronizeQueue(id);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SynchronizeQueue(id); });
 }
 
 void Tuner::SynchronizeQueues()
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SynchronizeQueues();
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SynchronizeQueues(); });
 }
 
 void Tuner::SynchronizeDevice()
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SynchronizeDevice();
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SynchronizeDevice(); });
 }
 
 void Tuner::SetProfilingCounters(const std::vector<std::string>& counters)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SetProfilingCounters(counters);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SetProfilingCounters(counters); });
 }
 
 void Tuner::SetCompilerOptions(const std::string& options, const bool overrideDefault)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SetCompilerOptions(options, overrideDefault);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SetCompilerOptions(options, overrideDefault); });
 }
 
 void Tuner::SetGlobalSizeType(const GlobalSizeType type)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SetGlobalSizeType(type);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SetGlobalSizeType(type); });
 }
 
 void Tuner::SetAutomaticGlobalSizeCorrection(const bool flag)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SetAutomaticGlobalSizeCorrection(flag);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SetAutomaticGlobalSizeCorrection(flag); });
 }
 
 void Tuner::SetKernelCacheCapacity(const uint64_t capacity)
 {
-    KTT_TRY
-    m_Tuner->GetComputeEngine().SetKernelCacheCapacity(capacity);
-    KTT_CATCH
+    Guarded([&]() { m_Tuner->GetComputeEngine().SetKernelCacheCapacity(capacity); });
 }
 
 std::vector<PlatformInfo> Tuner::GetPlatformInfo() const
 {
-    KTT_TRY
-    return m_Tuner->GetComputeEngine().GetPlatformInfo();
-    KTT_CATCH_RETURN(std::vector<PlatformInfo>{})
+    return Guarded(std::vector<PlatformInfo>{}, [&]() { return m_Tuner->GetComputeEngine().GetPlatformInfo(); });
 }
 
 std::vector<DeviceInfo> Tuner::GetDeviceInfo(const PlatformIndex platform) const
 {
-    KTT_TRY
-    return m_Tuner->GetComputeEngine().GetDeviceInfo(platform);
-    KTT_CATCH_RETURN(std::vector<DeviceInfo>{})
+    return Guarded(std::vector<DeviceInfo>{}, [&]() { return m_Tuner->GetComputeEngine().GetDeviceInfo(platform); });
 }
 
 DeviceInfo Tuner::GetCurrentDeviceInfo() const
 {
-    KTT_TRY
-    return m_Tuner->GetComputeEngine().GetCurrentDeviceInfo();
-    KTT_CATCH_RETURN(DeviceInfo())
+    return Guarded(DeviceInfo(), [&]() { return m_Tuner->GetComputeEngine().GetCurrentDeviceInfo(); });
 }
+
+// ---------------------------------------------------------------- logging
 
 void Tuner::SetLoggingLevel(const LoggingLevel level)
 {
@@ -539,42 +536,6 @@ void Tuner::SetLoggingTarget(std::ostream& outputTarget)
 void Tuner::SetLoggingTarget(const std::string& filePath)
 {
     Logger::GetLogger().SetLoggingTarget(filePath);
-}
-
-ArgumentId Tuner::AddArgumentWithReferencedData(const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation memoryLocation, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-    const ArgumentManagementType managementType, void* data, const size_t dataSize, const ArgumentId& customId)
-{
-    KTT_TRY
-    return m_Tuner->GetArgumentManager().AddArgumentWithReferencedData(elementSize, dataType, memoryLocation, accessType, memoryType,
-        managementType, data, dataSize, customId);
-    KTT_CATCH_RETURN(InvalidArgumentId)
-}
-
-ArgumentId Tuner::AddArgumentWithOwnedData(const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation memoryLocation, const ArgumentAccessType accessType, const ArgumentMemoryType memoryType,
-    const ArgumentManagementType managementType, const void* data, const size_t dataSize, const ArgumentId& customId)
-{
-    KTT_TRY
-    return m_Tuner->GetArgumentManager().AddArgumentWithOwnedData(elementSize, dataType, memoryLocation, accessType, memoryType,
-        managementType, data, dataSize, customId);
-    KTT_CATCH_RETURN(InvalidArgumentId)
-}
-
-ArgumentId Tuner::AddUserArgument(ComputeBuffer buffer, const size_t elementSize, const ArgumentDataType dataType,
-    const ArgumentMemoryLocation memoryLocation, const ArgumentAccessType accessType, const size_t dataSize, const ArgumentId& customId)
-{
-    KTT_TRY
-    return m_Tuner->GetArgumentManager().AddUserArgument(buffer, elementSize, dataType, memoryLocation, accessType, dataSize, customId);
-    KTT_CATCH_RETURN(InvalidArgumentId)
-}
-
-void Tuner::AddParameterInternal(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values,
-    const std::string& group)
-{
-    KTT_TRY
-    m_Tuner->GetKernelManager().AddParameter(id, name, values, group);
-    KTT_CATCH
 }
 
 } // namespace ktt

@@ -25,16 +25,16 @@ static std::unique_ptr<ComputeEngine> CreateEngine(const PlatformIndex platform,
 }
 
 TunerCore::TunerCore(const PlatformIndex platform, const DeviceIndex device, const ComputeApi api, const uint32_t queueCount) :
-    m_ArgumentManager(std::make_unique<KernelArgumentManager>()),
-    m_KernelManager(std::make_unique<KernelManager>(*m_ArgumentManager))
+    m_Arguments(std::make_unique<ArgumentRegistry>()),
+    m_Kernels(std::make_unique<KernelRegistry>(*m_Arguments))
 {
     m_ComputeEngine = CreateEngine(platform, device, api, queueCount);
     CreateDrivers();
 }
 
 TunerCore::TunerCore(const ComputeApi api, const ComputeApiInitializer& initializer, std::vector<QueueId>& assignedQueueIds) :
-    m_ArgumentManager(std::make_unique<KernelArgumentManager>()),
-    m_KernelManager(std::make_unique<KernelManager>(*m_ArgumentManager))
+    m_Arguments(std::make_unique<ArgumentRegistry>()),
+    m_Kernels(std::make_unique<KernelRegistry>(*m_Arguments))
 {
     if (api != ComputeApi::HIP)
     {
@@ -54,41 +54,41 @@ TunerCore::~TunerCore()
 
 void TunerCore::CreateDrivers()
 {
-    m_Executor = std::make_unique<Executor>(*m_ComputeEngine, *m_ArgumentManager);
+    m_Executor = std::make_unique<Executor>(*m_ComputeEngine, *m_Arguments);
     m_TuningLoop = std::make_unique<TuningLoop>(*m_Executor);
 }
 
 void TunerCore::RemoveKernel(const KernelId id)
 {
-    m_KernelManager->RemoveKernel(id);
+    m_Kernels->RemoveKernel(id);
     m_TuningLoop->Reset(id, true);
     m_Executor->ForgetKernel(id);
 }
 
 void TunerCore::RemoveArgument(const ArgumentId& id)
 {
-    if (m_KernelManager->IsArgumentUsed(id))
+    if (m_Kernels->UsesArgument(id))
     {
-        throw KttException("Argument with id " + id + " cannot be removed because it is still referenced by at least one kernel definition");
+        throw KttException("Argument " + id + " is still bound to a kernel definition");
     }
     m_ComputeEngine->ClearBuffer(id);
     m_Executor->Checker().Forget(id);
-    m_ArgumentManager->RemoveArgument(id);
+    m_Arguments->Remove(id);
 }
 
 std::string TunerCore::GetKernelSource(const KernelId id, const KernelConfiguration& configuration) const
 {
-    const auto& kernel = m_KernelManager->GetKernel(id);
+    const Kernel& kernel = m_Kernels->GetKernel(id);
     if (kernel.IsComposite())
     {
-        throw KttException("Kernel source can only be retrieved for a simple kernel");
+        throw KttException("Kernel " + kernel.GetName() + " has several definitions; ask for one definition's source");
     }
     return configuration.GeneratePrefix() + kernel.GetPrimaryDefinition().GetSource();
 }
 
 std::string TunerCore::GetKernelDefinitionSource(const KernelDefinitionId id, const KernelConfiguration& configuration) const
 {
-    return configuration.GeneratePrefix() + m_KernelManager->GetDefinition(id).GetSource();
+    return configuration.GeneratePrefix() + m_Kernels->GetDefinition(id).GetSource();
 }
 
 JsonValue TunerCore::CreateMetadata() const

@@ -32,9 +32,9 @@ public:
     explicit TunerCore(const ComputeApi api, const ComputeApiInitializer& initializer, std::vector<QueueId>& assignedQueueIds);
     ~TunerCore();
 
-    KernelArgumentManager& GetArgumentManager() { return *m_ArgumentManager; }
-    KernelManager& GetKernelManager() { return *m_KernelManager; }
-    const KernelManager& GetKernelManager() const { return *m_KernelManager; }
+    ArgumentRegistry& GetArguments() { return *m_Arguments; }
+    KernelRegistry& GetKernels() { return *m_Kernels; }
+    const KernelRegistry& GetKernels() const { return *m_Kernels; }
     ComputeEngine& GetComputeEngine() { return *m_ComputeEngine; }
     const ComputeEngine& GetComputeEngine() const { return *m_ComputeEngine; }
     Executor& GetExecutor() { return *m_Executor; }
@@ -54,8 +54,8 @@ public:
     static void Log(const LoggingLevel level, const std::string& message);
 
 private:
-    std::unique_ptr<KernelArgumentManager> m_ArgumentManager;
-    std::unique_ptr<KernelManager> m_KernelManager;
+    std::unique_ptr<ArgumentRegistry> m_Arguments;
+    std::unique_ptr<KernelRegistry> m_Kernels;
     std::unique_ptr<ComputeEngine> m_ComputeEngine;
     std::unique_ptr<Executor> m_Executor;
     std::unique_ptr<TuningLoop> m_TuningLoop;

@@ -51,33 +51,40 @@ std::vector<ParameterValue> Values(const std::vector<uint64_t>& values)
 
 /** The CoulombSum3d space of Examples/CoulombSum3d/CoulombSum3d.cpp:139-209 (default, neither
  *  dense nor wide parameters), for the CUDA or the OpenCL variant. */
-KernelId AddReferenceCoulomb3dSpace(KernelManager& manager, const bool openCl)
+KernelId AddReferenceCoulomb3dSpace(KernelRegistry& registry, const bool openCl)
 {
-    const KernelDefinitionId definition = manager.AddKernelDefinition(openCl ? "directCoulombSumCl" : "directCoulombSum", "",
+    const KernelDefinitionId definition = registry.AddDefinition(openCl ? "directCoulombSumCl" : "directCoulombSum", "",
         DimensionVector(256, 256, 256), DimensionVector(1, 1, 1), {});
-    const KernelId kernel = manager.CreateKernel(openCl ? "CoulombSumCl" : "CoulombSum", {definition});
-    manager.AddParameter(kernel, "WORK_GROUP_SIZE_X", Values({16, 32}), "");
-    manager.AddParameter(kernel, "WORK_GROUP_SIZE_Y", Values({1, 2, 4, 8}), "");
-    manager.AddParameter(kernel, "WORK_GROUP_SIZE_Z", Values({1}), "");
-    manager.AddParameter(kernel, "Z_ITERATIONS", Values({1, 2, 4, 8, 16, 32}), "");
-    manager.AddParameter(kernel, "INNER_UNROLL_FACTOR", Values({0, 1, 2, 4, 8, 16, 32}), "");
+    const KernelId id = registry.AddKernel(openCl ? "CoulombSumCl" : "CoulombSum", {definition});
+    Kernel& kernel = registry.GetKernel(id);
+    const auto parameter = [&kernel](const std::string& name, const std::vector<uint64_t>& values)
+    {
+        kernel.AddParameter(KernelParameter(name, Values(values), ""));
+    };
+    parameter("WORK_GROUP_SIZE_X", {16, 32});
+    parameter("WORK_GROUP_SIZE_Y", {1, 2, 4, 8});
+    parameter("WORK_GROUP_SIZE_Z", {1});
+    parameter("Z_ITERATIONS", {1, 2, 4, 8, 16, 32});
+    parameter("INNER_UNROLL_FACTOR", {0, 1, 2, 4, 8, 16, 32});
     if (openCl)
     {
-        manager.AddParameter(kernel, "USE_CONSTANT_MEMORY", Values({0, 1}), "");
-        manager.AddParameter(kernel, "USE_SOA", Values({0, 1}), "");
-        manager.AddParameter(kernel, "VECTOR_SIZE", Values({1, 2, 4, 8, 16}), "");
-        manager.AddConstraint(kernel, {"USE_SOA", "VECTOR_SIZE"}, [](const std::vector<uint64_t>& v) { return v[0] || v[1] == 1; });
+        parameter("USE_CONSTANT_MEMORY", {0, 1});
+        parameter("USE_SOA", {0, 1});
+        parameter("VECTOR_SIZE", {1, 2, 4, 8, 16});
+        kernel.AddConstraint(KernelConstraint({"USE_SOA", "VECTOR_SIZE"},
+            ConstraintFunction([](const std::vector<uint64_t>& v) { return v[0] || v[1] == 1; })));
     }
     else
     {
-        manager.AddParameter(kernel, "USE_CONSTANT_MEMORY", Values({0}), "");
-        manager.AddParameter(kernel, "USE_SOA", Values({0, 1}), "");
-        manager.AddParameter(kernel, "VECTOR_SIZE", Values({1}), "");
+        parameter("USE_CONSTANT_MEMORY", {0});
+        parameter("USE_SOA", {0, 1});
+        parameter("VECTOR_SIZE", {1});
     }
-    manager.AddConstraint(kernel, {"INNER_UNROLL_FACTOR", "Z_ITERATIONS"}, [](const std::vector<uint64_t>& v) { return v[0] < v[1]; });
-    manager.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y"},
-        [](const std::vector<uint64_t>& v) { return v[0] * v[1] >= 64; });
-    return kernel;
+    kernel.AddConstraint(KernelConstraint({"INNER_UNROLL_FACTOR", "Z_ITERATIONS"},
+        ConstraintFunction([](const std::vector<uint64_t>& v) { return v[0] < v[1]; })));
+    kernel.AddConstraint(KernelConstraint({"WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y"},
+        ConstraintFunction([](const std::vector<uint64_t>& v) { return v[0] * v[1] >= 64; })));
+    return id;
 }
 
 /** Canonical text of a configuration: pairs sorted by name. */
@@ -146,8 +153,8 @@ void TestPrefix()
 
 void TestSpaceSizes()
 {
-    KernelArgumentManager arguments;
-    KernelManager manager(arguments);
+    ArgumentRegistry arguments;
+    KernelRegistry manager(arguments);
     const KernelId cuda = AddReferenceCoulomb3dSpace(manager, false);
     const KernelId openCl = AddReferenceCoulomb3dSpace(manager, true);
     DeterministicSearcher first;
@@ -168,8 +175,8 @@ void TestSpaceSizes()
 void TestFixtureSpaceAndBest(const std::vector<KernelResult>& results)
 {
     CHECK(results.size() == 210);
-    KernelArgumentManager arguments;
-    KernelManager manager(arguments);
+    ArgumentRegistry arguments;
+    KernelRegistry manager(arguments);
     const KernelId kernel = AddReferenceCoulomb3dSpace(manager, false);
 
     // The dump holds exactly the space this tuner generates from the example's definitions.
@@ -210,8 +217,8 @@ void TestFixtureSpaceAndBest(const std::vector<KernelResult>& results)
 
 void TestRandomAndMcmc(const std::vector<KernelResult>& results)
 {
-    KernelArgumentManager arguments;
-    KernelManager manager(arguments);
+    ArgumentRegistry arguments;
+    KernelRegistry manager(arguments);
     const KernelId kernel = AddReferenceCoulomb3dSpace(manager, false);
     const Kernel& definition = manager.GetKernel(kernel);
 
@@ -275,8 +282,8 @@ void TestRandomAndMcmc(const std::vector<KernelResult>& results)
 
 void TestStopConditions(const std::vector<KernelResult>& results)
 {
-    KernelArgumentManager arguments;
-    KernelManager manager(arguments);
+    ArgumentRegistry arguments;
+    KernelRegistry manager(arguments);
     const Kernel& kernel = manager.GetKernel(AddReferenceCoulomb3dSpace(manager, false));
     {
         DeterministicSearcher searcher;
