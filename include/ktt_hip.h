@@ -184,6 +184,26 @@ int ktt_coulomb3d_tune(const float* host_atoms, int n_atoms, float spacing, int 
                        char* best_config, size_t best_config_length, double* best_ms,
                        int* valid_count, int* tested_count, float* energy_grid);
 
+/* Runs a KTT JSON tuning script whose KernelSpecification.Language is "HIP" (the job of the
+ * reference's KttTuningLauncher, TuningLoader/TuningLauncher.cpp, with the schema of
+ * TuningLoader/TuningSchema.h). Script expressions are evaluated without Python. Relative
+ * file names are relative to the script. output_file (optional, no extension) overrides
+ * General.OutputFile; configuration_limit > 0 replaces the script's Budget with a
+ * ConfigurationCount. Outputs, each optional: best_config (NAME=VALUE,...), space_size,
+ * tested, valid (configurations that ran and passed validation) and best_ms (kernel time of
+ * the best configuration). */
+int ktt_run_tuning_script(const char* script_path, const char* output_file, long long configuration_limit,
+                          char* best_config, size_t best_config_length, long long* space_size, long long* tested,
+                          long long* valid, double* best_ms);
+
+/* Number of configurations in a script's ConfigurationSpace (parameters and conditions);
+ * needs no device. */
+int ktt_tuning_script_space(const char* script_path, long long* size);
+
+/* Evaluates a script generator expression (a Python expression in the element index i, as in
+ * an argument's "Generator" DataSource) for i = 0 .. count-1 into values. */
+int ktt_script_generate(const char* expression, long long count, double* values);
+
 /* Dynamic tuning run of the compensated CoulombSum3d kernel with ktt::Tuner::TuneIteration
  * (replaces the online-tuning loop a user writes around Source/Tuner.h:662-698, whose
  * iteration is Source/TuningRunner/TuningRunner.cpp:84-128). The atoms are copied to the

@@ -55,7 +55,7 @@ KERNEL_CHECKS = [
     ("CoulombBatched.hip", "coulombBatchedReset", None),
 ]
 
-EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative"]
+EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative", "KttTuningLauncher"]
 
 
 def _run(command: list[str]) -> None:

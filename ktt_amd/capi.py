@@ -56,6 +56,11 @@ SIGNATURES = {
     "ktt_coulomb3d_tune": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_char_p,
                                       _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_double), _c.POINTER(_c.c_int),
                                       _c.POINTER(_c.c_int), _vp]),
+    "ktt_run_tuning_script": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_longlong, _c.c_char_p, _c.c_size_t,
+                                         _c.POINTER(_c.c_longlong), _c.POINTER(_c.c_longlong), _c.POINTER(_c.c_longlong),
+                                         _c.POINTER(_c.c_double)]),
+    "ktt_tuning_script_space": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_longlong)]),
+    "ktt_script_generate": (_c.c_int, [_c.c_char_p, _c.c_longlong, _vp]),
     "ktt_coulomb3d_dynamic": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_float, _c.c_ulonglong, _c.c_int,
                                          _c.c_int, _c.c_char_p, _vp, _vp, _vp, _c.c_char_p, _c.c_size_t,
                                          _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _vp, _vp]),
@@ -284,6 +289,30 @@ def coulomb3d_tune(host_atoms: np.ndarray, spacing: float, grid: int, searcher: 
            "ktt_coulomb3d_tune")
     return {"best_config": best.value.decode(), "best_ms": best_ms.value, "valid": valid.value, "tested": tested.value,
             "grid": output}
+
+
+def run_tuning_script(script_path: str, output_file: str | None = None, configuration_limit: int = 0) -> dict:
+    """Runs a KTT JSON tuning script (Language "HIP"); returns its summary."""
+    best = ctypes.create_string_buffer(2048)
+    space, tested, valid = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+    best_ms = ctypes.c_double(0.0)
+    _check(library().ktt_run_tuning_script(str(script_path).encode(), output_file.encode() if output_file else None,
+                                           configuration_limit, best, len(best), ctypes.byref(space), ctypes.byref(tested),
+                                           ctypes.byref(valid), ctypes.byref(best_ms)), "ktt_run_tuning_script")
+    return {"best_config": best.value.decode(), "space": space.value, "tested": tested.value, "valid": valid.value,
+            "best_ms": best_ms.value}
+
+
+def tuning_script_space(script_path: str) -> int:
+    size = ctypes.c_longlong(0)
+    _check(library().ktt_tuning_script_space(str(script_path).encode(), ctypes.byref(size)), "ktt_tuning_script_space")
+    return size.value
+
+
+def script_generate(expression: str, count: int) -> np.ndarray:
+    values = np.zeros(count, dtype=np.float64)
+    _check(library().ktt_script_generate(expression.encode(), count, values.ctypes.data_as(ctypes.c_void_p)), "ktt_script_generate")
+    return values
 
 
 def coulomb3d_dynamic(host_atoms: np.ndarray, spacing: float, grid: int, steps: int, sigma: float, seed: int,
