@@ -223,6 +223,25 @@ int ktt_coulomb3d_dynamic(const float* host_atoms, int n_atoms, float spacing, i
                           int* step_tuned, char* best_config, size_t best_config_length,
                           int* tested_count, int* space_size, float* final_atoms, float* final_grid);
 
+/* ---- 6. Searchers over saved results; hardware counters ------------------------------------ */
+
+/* Simulated tuning over saved KTT JSON results (Tuner::SimulateTuning, Source/Tuner.h:700-712,
+ * Source/TuningRunner/TuningRunner.cpp:131-201), without a device: the configuration space is
+ * the set of configurations the file holds and each proposal is answered with its saved
+ * result. searcher: "Deterministic", "Random", "MCMC" or "ProfileBased"
+ * (Source/Tuner.h:738-748); knowledge_file (optional) holds KTT JSON results with counters,
+ * the profile-based searcher's knowledge base; compute_units sizes its analysis (0: 256).
+ * seed 0 draws a seed. max_steps <= 0 runs until the searcher ends. step_ms[capacity] receives
+ * each step's total duration in milliseconds (-1 for an invalid result); steps the count. */
+int ktt_simulate_search(const char* results_file, const char* searcher, const char* knowledge_file,
+                        unsigned long long seed, long long max_steps, unsigned compute_units,
+                        double* step_ms, long long capacity, long long* steps);
+
+/* Enables hardware counters for profiled runs (ktt::EnableHipCounters; replaces the CUPTI
+ * set-up of Source/ComputeEngine/Cuda/Cupti/CuptiInstance.cpp). Must precede the process's
+ * first HIP call; KTT_HIP_COUNTERS=1 in the environment does the same when the library loads. */
+int ktt_hip_enable_counters(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,8 @@
 
 * ``ktt_amd/libktt_hip.so``: the ktt C++ API with the HIP compute engine and the C ABI of
   ``include/ktt_hip.h`` (g++, linked against the ROCm HIP runtime and hipRTC).
+* ``ktt_amd/libktt_hip_counters.so``: hardware counters through rocprofiler-sdk, loaded by the
+  main library only when counters are enabled (KTT_HIP_COUNTERS=1 or ktt::EnableHipCounters()).
 * ``ktt_amd/bin/``: the example programs (CoulombSum2d, CoulombSum3d, CoulombSum3dIterative)
   written against the ktt C++ API.
 * every kernel source under ``ktt_amd/kernels`` is compiled once for gfx950 through the
@@ -25,6 +27,7 @@ CSRC = PACKAGE / "csrc"
 KERNELS = PACKAGE / "kernels"
 OBJ_DIR = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libktt_hip.so"
+COUNTERS_LIBRARY = PACKAGE / "libktt_hip_counters.so"
 BIN_DIR = PACKAGE / "bin"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
@@ -81,6 +84,15 @@ def build_library(jobs: int = 8) -> Path:
     return LIBRARY
 
 
+def build_counters() -> Path:
+    temporary = COUNTERS_LIBRARY.with_suffix(".so.tmp")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", "-shared",
+          str(CSRC / "counters" / "HipCounters.cpp"), "-o", str(temporary), f"-L{ROCM / 'lib'}", "-lrocprofiler-sdk",
+          f"-Wl,-rpath,{ROCM / 'lib'}", "-pthread"])
+    os.replace(temporary, COUNTERS_LIBRARY)
+    return COUNTERS_LIBRARY
+
+
 def build_examples() -> list[Path]:
     BIN_DIR.mkdir(exist_ok=True)
     built = []
@@ -127,6 +139,7 @@ def build(verbose: bool = True) -> None:
     if verbose:
         print("[ktt_amd.build] compiling libktt_hip.so", flush=True)
     build_library()
+    build_counters()
     if verbose:
         print("[ktt_amd.build] compiling examples", flush=True)
     build_examples()

@@ -61,6 +61,9 @@ SIGNATURES = {
                                          _c.POINTER(_c.c_double)]),
     "ktt_tuning_script_space": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_longlong)]),
     "ktt_script_generate": (_c.c_int, [_c.c_char_p, _c.c_longlong, _vp]),
+    "ktt_simulate_search": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_ulonglong, _c.c_longlong, _c.c_uint, _vp,
+                                       _c.c_longlong, _c.POINTER(_c.c_longlong)]),
+    "ktt_hip_enable_counters": (_c.c_int, []),
     "ktt_coulomb3d_dynamic": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_float, _c.c_ulonglong, _c.c_int,
                                          _c.c_int, _c.c_char_p, _vp, _vp, _vp, _c.c_char_p, _c.c_size_t,
                                          _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _vp, _vp]),
@@ -313,6 +316,25 @@ def script_generate(expression: str, count: int) -> np.ndarray:
     values = np.zeros(count, dtype=np.float64)
     _check(library().ktt_script_generate(expression.encode(), count, values.ctypes.data_as(ctypes.c_void_p)), "ktt_script_generate")
     return values
+
+
+def simulate_search(results_file: str, searcher: str, knowledge_file: str | None = None, seed: int = 0, max_steps: int = 0,
+                    compute_units: int = 0) -> np.ndarray:
+    """Simulated tuning over saved KTT JSON results; the total duration (ms) of every step, -1
+    for invalid results."""
+    capacity = 1 << 20 if max_steps <= 0 else max_steps
+    durations = np.zeros(capacity, dtype=np.float64)
+    steps = ctypes.c_longlong(0)
+    _check(library().ktt_simulate_search(str(results_file).encode(), searcher.encode(),
+                                         str(knowledge_file).encode() if knowledge_file else None, seed, max_steps, compute_units,
+                                         durations.ctypes.data_as(ctypes.c_void_p), capacity, ctypes.byref(steps)),
+           "ktt_simulate_search")
+    return durations[:min(steps.value, capacity)].copy()
+
+
+def enable_counters() -> None:
+    """Enables hardware counters; effective only before the process's first HIP call."""
+    _check(library().ktt_hip_enable_counters(), "ktt_hip_enable_counters")
 
 
 def coulomb3d_dynamic(host_atoms: np.ndarray, spacing: float, grid: int, steps: int, sigma: float, seed: int,

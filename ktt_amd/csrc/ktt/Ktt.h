@@ -10,6 +10,8 @@
 
 #include <ComputeInterface.h>
 #include <DimensionVector.h>
+#include <HipCounters.h>
+#include <ProfileBasedSearcher.h>
 #include <Info.h>
 #include <KernelConfiguration.h>
 #include <KernelResult.h>

@@ -112,6 +112,12 @@ public:
     std::vector<KernelResult> SimulateTuning(const KernelId id, const std::vector<KernelResult>& results,
         std::unique_ptr<StopCondition> stopCondition = nullptr);
     void SetSearcher(const KernelId id, std::unique_ptr<Searcher> searcher);
+    /** Profile-based searcher (Source/Tuner.h:738-748, see ProfileBasedSearcher.h). `modelPath`
+     *  names KTT JSON results with counters of the same kernel space, the searcher's knowledge
+     *  base (empty: the searcher learns from its own profiled runs); pickled models are not read.
+     *  `useBuiltinModule` has no effect (the searcher is native code). */
+    void SetProfileBasedSearcher(const KernelId id, const std::string& modelPath, const bool useBuiltinModule = true,
+        const uint32_t batchSize = 5, const uint32_t neighborSize = 100, const uint32_t randomSize = 10);
     void InitializeConfigurationData(const KernelId id);
     void ClearConfigurationData(const KernelId id);
     uint64_t GetConfigurationsCount(const KernelId id) const;

@@ -17,12 +17,16 @@ std::string Exploration::Key(const std::vector<uint32_t>& row)
 }
 
 Exploration::Exploration(const Kernel& kernel, Searcher& searcher) :
+    Exploration(kernel, searcher, DefaultSearcherSeed())
+{}
+
+Exploration::Exploration(const Kernel& kernel, Searcher& searcher, const uint64_t seed) :
     m_Kernel(kernel),
     m_Searcher(searcher),
     m_Size(0),
     m_BestDuration(InvalidDuration),
     m_SearcherLive(true),
-    m_Random(DefaultSearcherSeed())
+    m_Random(seed)
 {
     const auto& parameters = kernel.GetParameters();
     std::map<std::string, std::vector<size_t>> byGroup;
@@ -476,7 +480,21 @@ std::vector<KernelResult> TuningLoop::Tune(const Kernel& kernel, const KernelDim
     std::vector<KernelResult> results;
     while (!exploration.Finished())
     {
-        KernelResult result = m_Executor.Execute(kernel, exploration.Proposal(), dimensions, KernelRunMode::OfflineTuning, {});
+        const KernelConfiguration configuration = exploration.Proposal();
+        KernelResult result = m_Executor.Execute(kernel, configuration, dimensions, KernelRunMode::OfflineTuning, {});
+        if (result.HasRemainingProfilingRuns())
+        {
+            // Counter passes: the same launch runs until every pass has been collected, and the
+            // last run carries the counters and the overheads of all runs (TuningRunner.cpp:37-54).
+            KernelResult passes(kernel.GetName(), configuration);
+            passes.FuseProfilingTimes(result, true);
+            while (result.IsValid() && result.HasRemainingProfilingRuns())
+            {
+                result = m_Executor.Execute(kernel, configuration, dimensions, KernelRunMode::OfflineTuning, {});
+                passes.FuseProfilingTimes(result, false);
+            }
+            result.CopyProfilingTimes(passes);
+        }
         result.SetSearcherOverhead(RunScopeTimer([&]() { exploration.Record(result); }));
         results.push_back(result);
         if (stop != nullptr)
@@ -510,7 +528,9 @@ KernelResult TuningLoop::Step(const Kernel& kernel, const KernelDimensions& dime
             + " for " + kernel.GetName());
     }
     KernelResult result = m_Executor.Execute(kernel, configuration, dimensions, mode, output);
-    if (searching)
+    // A profiled step with counter passes left keeps its configuration: the caller steps again
+    // until the last pass (TuningRunner.cpp:118).
+    if (searching && !result.HasRemainingProfilingRuns())
     {
         result.SetSearcherOverhead(RunScopeTimer([&]() { exploration.Record(result); }));
     }

@@ -42,6 +42,8 @@ class Exploration
 public:
     /** Enumerates the space and binds the searcher; throws if the space is empty. */
     Exploration(const Kernel& kernel, Searcher& searcher);
+    /** Same, with the seed of the draws of unexplored configurations given. */
+    Exploration(const Kernel& kernel, Searcher& searcher, const uint64_t seed);
     ~Exploration();
     Exploration(const Exploration&) = delete;
     Exploration& operator=(const Exploration&) = delete;

@@ -7,6 +7,7 @@
 #include <KttException.h>
 
 #include "Common.h"
+#include "HipCounterService.h"
 #include "HipEngine.h"
 #include "Kernel.h"
 
@@ -492,20 +493,13 @@ void HipEngine::ClearKernelData(const std::string& kernelName)
     m_KernelCache.remove_if([&kernelName](const auto& entry) { return entry.second->GetName() == kernelName; });
 }
 
-ComputationResult HipEngine::RunKernelWithProfiling(const KernelComputeData& data, const QueueId queueId)
+void HipEngine::AddStaticCounters(const KernelComputeData& data, const KernelCompilationData& compilation,
+    KernelProfilingData& profiling) const
 {
-    // Single-pass profiling: the launch is timed and annotated with the static resource
-    // usage of the code object and the derived occupancy; no counter replay is needed.
-    const ComputeActionId id = RunKernelAsync(data, queueId, false);
-    ComputationResult result = WaitForComputeAction(id);
-    auto profiling = std::make_unique<KernelProfilingData>();
-    const auto& compilation = result.GetCompilationData();
-    profiling->AddCounter(KernelProfilingCounter("kernel_duration_ns", ProfilingCounterType::UnsignedInt,
-        static_cast<uint64_t>(result.GetDuration())));
-    profiling->AddCounter(KernelProfilingCounter("vgpr_count", ProfilingCounterType::UnsignedInt, compilation.m_RegistersCount));
-    profiling->AddCounter(KernelProfilingCounter("lds_bytes", ProfilingCounterType::UnsignedInt, compilation.m_LocalMemorySize));
-    profiling->AddCounter(KernelProfilingCounter("scratch_bytes", ProfilingCounterType::UnsignedInt,
-        compilation.m_PrivateMemorySize));
+    // Resource usage of the code object and the occupancy it allows (waves per SIMD).
+    profiling.AddCounter(KernelProfilingCounter("vgpr_count", ProfilingCounterType::UnsignedInt, compilation.m_RegistersCount));
+    profiling.AddCounter(KernelProfilingCounter("lds_bytes", ProfilingCounterType::UnsignedInt, compilation.m_LocalMemorySize));
+    profiling.AddCounter(KernelProfilingCounter("scratch_bytes", ProfilingCounterType::UnsignedInt, compilation.m_PrivateMemorySize));
     const uint64_t threads = static_cast<uint64_t>(data.GetLocalSize().GetTotalSize());
     const uint64_t wavesPerGroup = (threads + 63) / 64;
     const uint64_t vgprs = std::max<uint64_t>(compilation.m_RegistersCount, 1);
@@ -514,24 +508,143 @@ ComputationResult HipEngine::RunKernelWithProfiling(const KernelComputeData& dat
     const uint64_t groupsByLds = ldsPerGroup == 0 ? 64 : static_cast<uint64_t>(m_Properties.maxSharedMemoryPerMultiProcessor) / ldsPerGroup;
     const uint64_t wavesByLds = groupsByLds * wavesPerGroup / 4;
     const uint64_t wavesPerSimd = std::max<uint64_t>(1, std::min(wavesByVgpr, std::max<uint64_t>(wavesByLds, 1)));
-    profiling->AddCounter(KernelProfilingCounter("occupancy_waves_per_simd", ProfilingCounterType::UnsignedInt, wavesPerSimd));
+    profiling.AddCounter(KernelProfilingCounter("occupancy_waves_per_simd", ProfilingCounterType::UnsignedInt, wavesPerSimd));
+}
+
+ComputationResult HipEngine::RunWithStaticProfile(const KernelComputeData& data, const QueueId queueId)
+{
+    const ComputeActionId id = RunKernelAsync(data, queueId, false);
+    ComputationResult result = WaitForComputeAction(id);
+    auto profiling = std::make_unique<KernelProfilingData>();
+    profiling->AddCounter(KernelProfilingCounter("kernel_duration_ns", ProfilingCounterType::UnsignedInt,
+        static_cast<uint64_t>(result.GetDuration())));
+    AddStaticCounters(data, result.GetCompilationData(), *profiling);
     result.SetProfilingData(std::move(profiling));
+    return result;
+}
+
+void HipEngine::EnsurePlan(const std::vector<std::string>& counters)
+{
+    if (counters != m_PlannedCounters)
+    {
+        HipCounterService::Get().Plan(m_DeviceIndex, counters, m_PassOf);
+        m_PlannedCounters = counters;
+    }
+}
+
+ComputationResult HipEngine::RunKernelWithProfiling(const KernelComputeData& data, const QueueId queueId)
+{
+    HipCounterService& service = HipCounterService::Get();
+    if (!service.IsRunning())
+    {
+        if (!m_ProfilingCounters.empty())
+        {
+            throw KttException("Profiling counters were requested but " + service.UnavailableReason());
+        }
+        // Without the counter service a profiled run is a single timed run annotated with the
+        // static resource usage of the code object.
+        return RunWithStaticProfile(data, queueId);
+    }
+
+    const KernelComputeId id = data.GetUniqueIdentifier();
+    auto found = m_ProfilingSessions.find(id);
+    if (found == m_ProfilingSessions.end())
+    {
+        ProfilingSession session;
+        session.counters = m_ProfilingCounters.empty() ? HipCounterService::DefaultCounters() : m_ProfilingCounters;
+        EnsurePlan(session.counters);
+        session.passes = 1 + *std::max_element(m_PassOf.begin(), m_PassOf.end());
+        session.values.assign(session.counters.size(), 0.0);
+        found = m_ProfilingSessions.emplace(id, std::move(session)).first;
+    }
+    ProfilingSession& session = found->second;
+
+    if (session.runsDone == 0)
+    {
+        // The timing run: the kernel's duration is taken without counter collection.
+        ComputationResult result = WaitForComputeAction(RunKernelAsync(data, queueId, false));
+        session.duration = result.GetDuration();
+        session.runsDone = 1;
+        result.SetProfilingData(std::make_unique<KernelProfilingData>(static_cast<uint64_t>(session.passes)));
+        return result;
+    }
+
+    const size_t pass = session.runsDone - 1;
+    ComputationResult result;
+    try
+    {
+        EnsurePlan(session.counters);
+        const auto& local = data.GetLocalSize();
+        service.Arm(pass, static_cast<unsigned>(local.GetSizeX()), static_cast<unsigned>(local.GetSizeY()),
+            static_cast<unsigned>(local.GetSizeZ()));
+        result = WaitForComputeAction(RunKernelAsync(data, queueId, false));
+        std::vector<double> values;
+        service.Collect(values, 10000);
+        for (size_t i = 0; i < session.counters.size(); ++i)
+        {
+            if (m_PassOf[i] == pass)
+            {
+                session.values[i] = values[i];
+            }
+        }
+    }
+    catch (const KttException&)
+    {
+        service.Disarm();
+        m_ProfilingSessions.erase(found);
+        throw;
+    }
+    ++session.runsDone;
+
+    const uint64_t remaining = static_cast<uint64_t>(session.passes + 1 - session.runsDone);
+    if (remaining > 0)
+    {
+        result.SetProfilingData(std::make_unique<KernelProfilingData>(remaining));
+        return result;
+    }
+    auto profiling = std::make_unique<KernelProfilingData>();
+    for (size_t i = 0; i < session.counters.size(); ++i)
+    {
+        profiling->AddCounter(KernelProfilingCounter(session.counters[i], ProfilingCounterType::Double, session.values[i]));
+    }
+    profiling->AddCounter(KernelProfilingCounter("kernel_duration_ns", ProfilingCounterType::UnsignedInt,
+        static_cast<uint64_t>(session.duration)));
+    AddStaticCounters(data, result.GetCompilationData(), *profiling);
+    // The run reports the duration of the timing run; counter passes serialise the kernel.
+    result.SetDurationData(session.duration, result.GetOverhead(), result.GetCompilationOverhead());
+    result.SetProfilingData(std::move(profiling));
+    m_ProfilingSessions.erase(found);
     return result;
 }
 
 void HipEngine::SetProfilingCounters(const std::vector<std::string>& counters)
 {
+    HipCounterService& service = HipCounterService::Get();
+    if (!counters.empty())
+    {
+        if (!service.IsRunning())
+        {
+            throw KttException("Profiling counters cannot be collected: " + service.UnavailableReason());
+        }
+        EnsurePlan(counters);
+    }
     m_ProfilingCounters = counters;
+    m_ProfilingSessions.clear();
 }
 
-bool HipEngine::IsProfilingSessionActive([[maybe_unused]] const KernelComputeId& id)
+bool HipEngine::IsProfilingSessionActive(const KernelComputeId& id)
 {
-    return false;
+    return m_ProfilingSessions.count(id) != 0;
 }
 
-uint64_t HipEngine::GetRemainingProfilingRuns([[maybe_unused]] const KernelComputeId& id)
+uint64_t HipEngine::GetRemainingProfilingRuns(const KernelComputeId& id)
 {
-    return 0;
+    const auto found = m_ProfilingSessions.find(id);
+    if (found == m_ProfilingSessions.end())
+    {
+        return 0;
+    }
+    return static_cast<uint64_t>(found->second.passes + 1 - found->second.runsDone);
 }
 
 bool HipEngine::HasAccurateRemainingProfilingRuns() const

@@ -181,9 +181,25 @@ private:
     GlobalSizeType m_GlobalSizeType;
     bool m_GlobalSizeCorrection;
     bool m_Profiling;
-    std::vector<std::string> m_ProfilingCounters;
+    std::vector<std::string> m_ProfilingCounters;  // empty: the service's default counters
+
+    // Multi-pass collection of one launch: a timing run, then one run per counter pass.
+    struct ProfilingSession
+    {
+        std::vector<std::string> counters;
+        size_t passes = 0;
+        size_t runsDone = 0;           // 0 before the timing run
+        Nanoseconds duration = 0;      // of the timing run
+        std::vector<double> values;    // indexed like counters
+    };
+    std::map<KernelComputeId, ProfilingSession> m_ProfilingSessions;
+    std::vector<std::string> m_PlannedCounters;  // counter list of the service's current plan
+    std::vector<size_t> m_PassOf;
 
     void Initialize();
+    ComputationResult RunWithStaticProfile(const KernelComputeData& data, const QueueId queueId);
+    void AddStaticCounters(const KernelComputeData& data, const KernelCompilationData& compilation, KernelProfilingData& profiling) const;
+    void EnsurePlan(const std::vector<std::string>& counters);
     std::shared_ptr<HipKernel> LoadKernel(const KernelComputeData& data, bool& compiled);
     std::vector<std::string> BuildOptions(const KernelComputeData& data) const;
     Buffer& GetBuffer(const ArgumentId& id);

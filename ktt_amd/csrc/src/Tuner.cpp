@@ -364,6 +364,34 @@ std::vector<KernelResult> Tuner::SimulateTuning(const KernelId id, const std::ve
     });
 }
 
+void Tuner::SetProfileBasedSearcher(const KernelId id, const std::string& modelPath, [[maybe_unused]] const bool useBuiltinModule,
+    const uint32_t batchSize, const uint32_t neighborSize, const uint32_t randomSize)
+{
+    Guarded([&]()
+    {
+        std::vector<KernelResult> knowledge;
+        if (!modelPath.empty())
+        {
+            auto endsWith = [&](const std::string& suffix)
+            {
+                return modelPath.size() >= suffix.size() && modelPath.compare(modelPath.size() - suffix.size(), suffix.size(), suffix) == 0;
+            };
+            if (endsWith(".sav"))
+            {
+                throw KttException("Pickled models are not loaded: give KTT JSON results with profiling counters as " + modelPath);
+            }
+            const std::string path = endsWith(".json") ? modelPath.substr(0, modelPath.size() - 5) : modelPath;
+            UserData ignored;
+            knowledge = m_Tuner->LoadResults(path, OutputFormat::JSON, ignored);
+        }
+        ProfileSearcherOptions options;
+        options.batchSize = batchSize;
+        options.neighbourCount = neighborSize;
+        options.randomCount = randomSize;
+        SetSearcher(id, std::make_unique<ProfileBasedSearcher>(*this, options, knowledge));
+    });
+}
+
 void Tuner::SetSearcher(const KernelId id, std::unique_ptr<Searcher> searcher)
 {
     Guarded([&]()

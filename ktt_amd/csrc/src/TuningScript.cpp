@@ -527,6 +527,28 @@ TuningScriptSummary RunTuningScript(const std::string& scriptPath, const TuningS
     {
         tuner.SetSearcher(kernel, std::make_unique<McmcSearcher>());
     }
+    else if (searcher == "ProfileBased")
+    {
+        // Attributes as in TuningLoader/Commands/SearcherCommand.cpp:27-41; modelPath names KTT
+        // JSON results with counters (relative to the script).
+        std::map<std::string, std::string> attributes;
+        const JsonValue& search = script.At("Search");
+        if (search.Contains("Attributes"))
+        {
+            const JsonValue& list = search.At("Attributes");
+            for (size_t k = 0; k < list.Size(); ++k)
+            {
+                attributes[list[k].At("Name").AsString()] = list[k].At("Value").AsString();
+            }
+        }
+        auto number = [&](const char* name, const uint32_t fallback)
+        {
+            return attributes.count(name) != 0 ? static_cast<uint32_t>(std::stoul(attributes[name])) : fallback;
+        };
+        const std::string model = attributes.count("modelPath") != 0 && !attributes["modelPath"].empty()
+            ? Resolve(directory, attributes["modelPath"]) : std::string();
+        tuner.SetProfileBasedSearcher(kernel, model, true, number("batchSize", 5), number("neighborSize", 100), number("randomSize", 10));
+    }
     else if (searcher != "Deterministic")
     {
         throw KttException("Searcher " + searcher + " is not available in this build");

@@ -1,7 +1,7 @@
 // Host-side tests of the ktt core (no device needed): configuration prefix, configuration
 // spaces of the reference's Coulomb examples, the results loader on the reference's own RTX
-// 2080 Ti full-search dump, simulated tuning with the three searchers, stop conditions and the
-// JSON round trip.
+// 2080 Ti full-search dump, simulated tuning with the three searchers, stop conditions, the
+// JSON round trip, and the profile-based searcher's bottleneck analysis and candidate scoring.
 //
 // Usage: ktt_core_tests <path to tests/golden/coulomb3d_2080_space.json>
 // Exit status 0 when every check passes; each failure is printed with its line.
@@ -332,6 +332,78 @@ void TestJsonRoundTrip(const std::vector<KernelResult>& results)
     }
 }
 
+bool Near(const double a, const double b)
+{
+    return std::fabs(a - b) <= 1e-12 * std::max(1.0, std::fabs(b));
+}
+
+/** Bottleneck analysis, counter changes and candidate scoring of the profile-based searcher on
+ *  hand-made counter sets whose results follow by hand. */
+void TestProfileSearcherAnalysis()
+{
+    ProfileSearcherOptions options;
+    options.computeUnits = 68;
+
+    // CUDA metrics: 60 % DRAM throughput split 3:1 between reads and writes, L2 40 % split 1:1.
+    std::map<std::string, double> cuda{{"dram__throughput.avg.pct_of_peak_sustained_elapsed", 60.0},
+        {"dram__sectors_read.sum", 300.0}, {"dram__sectors_write.sum", 100.0},
+        {"lts__t_sectors.avg.pct_of_peak_sustained_elapsed", 40.0}, {"lts__t_sectors_op_read.sum", 50.0},
+        {"lts__t_sectors_op_write.sum", 50.0}, {"sm__warps_active.avg.pct_of_peak_sustained_active", 25.0},
+        {"smsp__cycles_active.avg.pct_of_peak_sustained_elapsed", 90.0}, {"smsp__issue_active.avg.pct_of_peak_sustained_active", 40.0},
+        {"smsp__inst_executed.sum", 100.0}, {"smsp__thread_inst_executed_per_inst_executed.ratio", 32.0},
+        {"smsp__thread_inst_executed_per_inst_executed.pct", 100.0}, {"smsp__sass_thread_inst_executed_op_fp32_pred_on.sum", 3200.0},
+        {"Global size", 68.0 * 64.0 * 5.0 / 2.0}, {"Local size", 256.0}};
+    auto bn = ProfileBasedSearcher::AnalyzeBottlenecks(cuda, options);
+    CHECK(Near(bn["bnDRAMRead"], 0.45));
+    CHECK(Near(bn["bnDRAMWrite"], 0.15));
+    CHECK(Near(bn["bnL2Read"], 0.2));
+    CHECK(Near(bn["bnL2Write"], 0.2));
+    CHECK(Near(bn["bnMPparal"], 0.75));
+    CHECK(Near(bn["bnGparal"], 0.1));
+    CHECK(Near(bn["bnThreads"], 0.5));
+    // fp32 is every executed thread instruction and issue is at 80 % of its dual-issue ceiling.
+    CHECK(Near(bn["bnSP"], 0.8));
+    CHECK(Near(bn["bnInstIssue"], 0.6 * (1.0 - 0.7) / 0.3));
+    auto changes = ProfileBasedSearcher::ComputeChanges(bn, cuda, options);
+    CHECK(Near(changes["dram__sectors_read.sum"], -0.45));
+    CHECK(Near(changes["smsp__sass_thread_inst_executed_op_fp32_pred_on.sum"], -0.8));
+    CHECK(Near(changes["Global size"], 0.5));
+
+    // CDNA counters: 4 GiB of HBM traffic (3:1) in 1 ms is 4.295e12 B/s, 53.7 % of 8 TB/s.
+    std::map<std::string, double> cdna{{"FETCH_SIZE", 3.0 * 1024 * 1024}, {"WRITE_SIZE", 1.0 * 1024 * 1024}, {"VALUBusy", 90.0},
+        {"F32_INSTS", 600.0}, {"SQ_INSTS_VALU", 1000.0}, {"SQ_INSTS_VALU_TRANS_F32", 200.0}, {"OccupancyPercent", 40.0},
+        {"SIMD_UTILIZATION", 0.95}, {"LdsUtil", 30.0}, {"kernel_duration_ns", 1.0e6}, {"Global size", 1.0e6}, {"Local size", 256.0}};
+    options.computeUnits = 256;
+    bn = ProfileBasedSearcher::AnalyzeBottlenecks(cdna, options);
+    const double hbm = 4.0 * 1024 * 1024 * 1024 / 1.0e-3 / 8.0e12;
+    CHECK(Near(bn["bnDRAMRead"], 0.75 * hbm));
+    CHECK(Near(bn["bnDRAMWrite"], 0.25 * hbm));
+    CHECK(Near(bn["bnSP"], 0.6 * 0.9));
+    CHECK(Near(bn["bnSFU"], 0.8 * 0.9));
+    CHECK(Near(bn["bnMPparal"], 0.6));
+    CHECK(Near(bn["bnGparal"], 0.05));
+    CHECK(Near(bn["bnSMRead"], 0.3));
+    // The transcendental unit, busy 0.72 of the time, is the class above the 0.7 threshold.
+    CHECK(Near(bn["bnInstIssue"], 0.1 * (0.72 - 0.7) / 0.3));
+    changes = ProfileBasedSearcher::ComputeChanges(bn, cdna, options);
+    CHECK(Near(changes["SQ_INSTS_VALU_TRANS_F32"], -(0.72 - 0.7) / 0.3));
+    CHECK(changes.count("F32_INSTS") == 0);  // 0.54 is below the threshold
+
+    // Scoring: one counter that should shrink. The candidate that halves it scores +1/3, the
+    // one that doubles it -1/3 (below the -0.25 cut-off), one without counters 0.
+    std::map<std::string, double> want{{"A", -1.0}};
+    std::map<std::string, double> current{{"A", 10.0}};
+    std::map<std::string, double> smaller{{"A", 5.0}};
+    std::map<std::string, double> larger{{"A", 20.0}};
+    std::map<std::string, double> slightlyLarger{{"A", 11.0}};
+    auto weights = ProfileBasedSearcher::ScoreCandidates(want, current, {&smaller, &larger, nullptr}, options);
+    CHECK(weights.size() == 3 && Near(weights[0], 256.0) && Near(weights[1], 0.0001) && Near(weights[2], 0.0001));
+    // Above the cut-off a negative score maps to 1 - score / lowest before the exponent.
+    weights = ProfileBasedSearcher::ScoreCandidates(want, current, {&smaller, &slightlyLarger, &larger}, options);
+    const double mild = -1.0 / 21.0;
+    CHECK(Near(weights[1], std::pow(1.0 - mild / (-1.0 / 3.0), 8.0)));
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -350,6 +422,7 @@ int main(int argc, char** argv)
     TestRandomAndMcmc(results);
     TestStopConditions(results);
     TestJsonRoundTrip(results);
+    TestProfileSearcherAnalysis();
     std::printf("%d checks, %d failures\n", g_Checks, g_Failures);
     return g_Failures == 0 ? 0 : 1;
 }
