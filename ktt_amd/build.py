@@ -58,7 +58,7 @@ KERNEL_CHECKS = [
     ("CoulombBatched.hip", "coulombBatchedReset", None),
 ]
 
-EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative", "KttTuningLauncher"]
+EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative", "KttTuningLauncher", "ProfiledCoulombSum3d"]
 
 
 def _run(command: list[str]) -> None:
