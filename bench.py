@@ -232,6 +232,51 @@ def batched_leg(capi, workloads, config=None, repeats: int = 3) -> dict:
     }
 
 
+NBODY_BODIES = 128 * 1024  # Examples/Nbody/Nbody.cpp:60
+NBODY_FLOP_PER_INTERACTION = 20  # the usual N-body count (3 sub, 3 FMA, rsqrt, 3 mul, 3 FMA)
+NBODY_CONFIGS = [
+    "WORK_GROUP_SIZE_X=256,WORK_GROUP_SIZE_Y=1,BODIES_PER_THREAD=2,ATOM_TILE=256,INNER_UNROLL_FACTOR=4",
+    "WORK_GROUP_SIZE_X=256,WORK_GROUP_SIZE_Y=1,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=4",
+    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=4",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=8",
+    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=1,BODIES_PER_THREAD=8,ATOM_TILE=512,INNER_UNROLL_FACTOR=4",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=2,BODIES_PER_THREAD=8,ATOM_TILE=256,INNER_UNROLL_FACTOR=8",
+]
+
+
+def nbody_leg(capi, steps: int, warmup: int) -> dict:
+    """One N-body step of the example's 128K bodies (positions, masses, velocities U[0,20), seeded)
+    for each of a few configurations; the fastest is reported."""
+    rng = np.random.default_rng(20240812)
+    n = NBODY_BODIES
+    pos = rng.uniform(0.0, 20.0, (n, 4)).astype(np.float32)
+    vel = np.zeros((n, 4), dtype=np.float32)
+    vel[:, :3] = rng.uniform(0.0, 20.0, (n, 3))
+    buffers = [capi.DeviceBuffer.from_array(pos), capi.DeviceBuffer.from_array(vel), capi.DeviceBuffer(16 * n),
+               capi.DeviceBuffer(16 * n)]
+    timings = {}
+    for config in NBODY_CONFIGS:
+        run = lambda: capi.nbody_step(buffers[0], buffers[1], n, 0.001, 0.5, 0.01, buffers[2], buffers[3], config=config)
+        per_launch, _ = timed_loop(capi, run, steps, warmup)
+        timings[config] = per_launch
+    for buffer in buffers:
+        buffer.close()
+    best = min(timings, key=timings.get)
+    interactions = float(n) * n
+    value = interactions / (timings[best] * 1e-3)
+    return {
+        "workload": f"N-body step, {n} bodies (Examples/Nbody), FP32",
+        "bodies": n,
+        "ms": timings[best],
+        "config": best,
+        "value": value,
+        "unit": "interactions/s",
+        "tflops": value * NBODY_FLOP_PER_INTERACTION / 1e12,
+        "frac_of_fp32_peak": value * NBODY_FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
+        "sweep_ms": timings,
+    }
+
+
 def traffic_summary() -> tuple[float | None, str | None]:
     if not TRAFFIC_SUMMARY.exists():
         return None, None
@@ -330,6 +375,8 @@ def main() -> int:
         progress("precision workload done")
         extras["batched"] = batched_leg(capi, workloads, config=args.batch_config)
         progress(f"batched: graph {extras['batched']['graph_ms']:.1f} ms, per system {extras['batched']['per_system_plain_ms']:.1f} ms")
+        extras["nbody"] = nbody_leg(capi, max(args.steps, 5), args.warmup)
+        progress(f"nbody: {extras['nbody']['ms']:.3f} ms, {extras['nbody']['tflops']:.1f} TFLOP/s")
 
     headline = results["compensated"]
     achieved = headline["value"] * FLOP_PER_INTERACTION / 1e12

@@ -242,6 +242,33 @@ int ktt_simulate_search(const char* results_file, const char* searcher, const ch
  * first HIP call; KTT_HIP_COUNTERS=1 in the environment does the same when the library loads. */
 int ktt_hip_enable_counters(void);
 
+/* ---- 7. N-body step (sibling kernel of the Coulomb sum) ------------------------------------ */
+
+/* One integration step of n bodies on the device (Examples/Nbody/Nbody.cu:271-410, kernel
+ * nbody_kernel): a_i = sum_j m_j d_ij / (|d_ij|^2 + softening_sqr)^(3/2),
+ * p' = p + dt v + damping dt^2 a, v' = v + dt a. body_info and new_body_info are n float4
+ * (x, y, z, mass); velocity and new_velocity n float4 (vx, vy, vz, 0). config as in section 3
+ * (WORK_GROUP_SIZE_X/Y, BODIES_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR; NULL: defaults); n
+ * must be a multiple of WORK_GROUP_SIZE_X * BODIES_PER_THREAD and ATOM_TILE * WORK_GROUP_SIZE_Y. */
+int ktt_nbody_step(const float* body_info, const float* velocity, int n, float time_delta, float damping,
+                   float softening_sqr, float* new_body_info, float* new_velocity, const char* config, void* stream,
+                   float* kernel_ms);
+
+/* The same step with the reference kernel (Examples/Nbody/NbodyReference.cu:40-75); n a
+ * multiple of 64. */
+int ktt_nbody_step_reference(const float* body_info, const float* velocity, int n, float time_delta, float damping,
+                             float softening_sqr, float* new_body_info, float* new_velocity, void* stream,
+                             float* kernel_ms);
+
+/* The N-body example's tuning run (Examples/Nbody/Nbody.cpp:146-256) through ktt::Tuner on
+ * host data: the kernel's space, both outputs validated against the reference kernel with
+ * SideBySideComparison at tolerance (the example uses 0.001). Searcher and outputs as for
+ * ktt_coulomb3d_tune; best_ms is the best configuration's kernel time. */
+int ktt_nbody_tune(const float* host_body_info, const float* host_velocity, int n, float time_delta, float damping,
+                   float softening_sqr, int searcher, int max_configurations, double tolerance,
+                   const char* results_file, char* best_config, size_t best_config_length, double* best_ms,
+                   int* valid_count, int* tested_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,13 @@ SIGNATURES = {
     "ktt_simulate_search": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_ulonglong, _c.c_longlong, _c.c_uint, _vp,
                                        _c.c_longlong, _c.POINTER(_c.c_longlong)]),
     "ktt_hip_enable_counters": (_c.c_int, []),
+    "ktt_nbody_step": (_c.c_int, [_vp, _vp, _c.c_int, _c.c_float, _c.c_float, _c.c_float, _vp, _vp, _c.c_char_p, _vp,
+                                  _c.POINTER(_c.c_float)]),
+    "ktt_nbody_step_reference": (_c.c_int, [_vp, _vp, _c.c_int, _c.c_float, _c.c_float, _c.c_float, _vp, _vp, _vp,
+                                            _c.POINTER(_c.c_float)]),
+    "ktt_nbody_tune": (_c.c_int, [_vp, _vp, _c.c_int, _c.c_float, _c.c_float, _c.c_float, _c.c_int, _c.c_int, _c.c_double,
+                                  _c.c_char_p, _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_double), _c.POINTER(_c.c_int),
+                                  _c.POINTER(_c.c_int)]),
     "ktt_coulomb3d_dynamic": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_float, _c.c_ulonglong, _c.c_int,
                                          _c.c_int, _c.c_char_p, _vp, _vp, _vp, _c.c_char_p, _c.c_size_t,
                                          _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _vp, _vp]),
@@ -330,6 +337,38 @@ def simulate_search(results_file: str, searcher: str, knowledge_file: str | None
                                          durations.ctypes.data_as(ctypes.c_void_p), capacity, ctypes.byref(steps)),
            "ktt_simulate_search")
     return durations[:min(steps.value, capacity)].copy()
+
+
+def nbody_step(bodies: DeviceBuffer, velocity: DeviceBuffer, n: int, dt: float, damping: float, softening_sqr: float,
+               new_bodies: DeviceBuffer, new_velocity: DeviceBuffer, config=None, stream=None, timed: bool = False):
+    ms = _timing(timed)
+    _check(library().ktt_nbody_step(bodies.pointer, velocity.pointer, n, dt, damping, softening_sqr, new_bodies.pointer,
+                                    new_velocity.pointer, _config(config), stream, ms), "ktt_nbody_step")
+    return ms.value if timed else None
+
+
+def nbody_step_reference(bodies: DeviceBuffer, velocity: DeviceBuffer, n: int, dt: float, damping: float, softening_sqr: float,
+                         new_bodies: DeviceBuffer, new_velocity: DeviceBuffer, stream=None, timed: bool = False):
+    ms = _timing(timed)
+    _check(library().ktt_nbody_step_reference(bodies.pointer, velocity.pointer, n, dt, damping, softening_sqr, new_bodies.pointer,
+                                              new_velocity.pointer, stream, ms), "ktt_nbody_step_reference")
+    return ms.value if timed else None
+
+
+def nbody_tune(host_bodies: np.ndarray, host_velocity: np.ndarray, dt: float, damping: float, softening_sqr: float,
+               searcher: int = SEARCHER_DETERMINISTIC, max_configurations: int = 0, tolerance: float = 0.001,
+               results_file: str | None = None) -> dict:
+    bodies = np.ascontiguousarray(host_bodies, dtype=np.float32)
+    velocity = np.ascontiguousarray(host_velocity, dtype=np.float32)
+    best = ctypes.create_string_buffer(1024)
+    best_ms = ctypes.c_double(0.0)
+    valid = ctypes.c_int(0)
+    tested = ctypes.c_int(0)
+    _check(library().ktt_nbody_tune(bodies.ctypes.data_as(ctypes.c_void_p), velocity.ctypes.data_as(ctypes.c_void_p), bodies.shape[0],
+                                    dt, damping, softening_sqr, searcher, max_configurations, tolerance,
+                                    results_file.encode() if results_file else None, best, len(best), ctypes.byref(best_ms),
+                                    ctypes.byref(valid), ctypes.byref(tested)), "ktt_nbody_tune")
+    return {"best_config": best.value.decode(), "best_ms": best_ms.value, "valid": valid.value, "tested": tested.value}
 
 
 def enable_counters() -> None:

@@ -1,0 +1,167 @@
+// N-body integration step for gfx950: the counterpart of Examples/Nbody/Nbody.cu:271-410
+// (same kernel name and arguments), with its own decomposition.
+//
+//   a_i  = sum_j m_j (p_j - p_i) / (|p_j - p_i|^2 + eps^2)^(3/2)
+//   p_i' = p_i + dt v_i + damping dt^2 a_i,   v_i' = v_i + dt a_i
+//
+// Decomposition. A workgroup of WORK_GROUP_SIZE_X x WORK_GROUP_SIZE_Y threads owns
+// WORK_GROUP_SIZE_X * BODIES_PER_THREAD target bodies; thread x holds bodies
+// base + b * WORK_GROUP_SIZE_X + x (b < BODIES_PER_THREAD) in registers, so every load and store
+// of a body index is coalesced across the wave. Row y of the workgroup sums the sources of
+// every WORK_GROUP_SIZE_Y-th tile of ATOM_TILE bodies; the rows' partial accelerations are added
+// in row order through LDS at the end. A tile is staged in LDS as float4 (x, y, z, m) and read
+// by all lanes of a wave at the same address (a broadcast, no bank conflict).
+//
+// Per interaction and target body: 3 subtractions, 3 FMAs for r^2 + eps^2, one v_rsq_f32,
+// 3 multiplications for m r^-3 and 3 FMAs into the acceleration: 12 VALU + 1 transcendental.
+//
+// Requirements (checked by the host and the tuning space's constraints): n is a multiple of
+// WORK_GROUP_SIZE_X * BODIES_PER_THREAD and of ATOM_TILE * WORK_GROUP_SIZE_Y.
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+
+#ifndef WORK_GROUP_SIZE_X
+#define WORK_GROUP_SIZE_X 256
+#endif
+#ifndef WORK_GROUP_SIZE_Y
+#define WORK_GROUP_SIZE_Y 1
+#endif
+#ifndef BODIES_PER_THREAD
+#define BODIES_PER_THREAD 2
+#endif
+#ifndef ATOM_TILE
+#define ATOM_TILE 256
+#endif
+#ifndef INNER_UNROLL_FACTOR
+#define INNER_UNROLL_FACTOR 4
+#endif
+#ifndef USE_SOA
+#define USE_SOA 0
+#endif
+
+#define KTT_THREADS (WORK_GROUP_SIZE_X * WORK_GROUP_SIZE_Y)
+
+static_assert(ATOM_TILE % INNER_UNROLL_FACTOR == 0, "the unroll factor divides the tile");
+
+extern "C" __global__ __launch_bounds__(KTT_THREADS) void nbody_kernel(float timeDelta,
+    const float4* __restrict__ oldBodyInfo, const float* __restrict__ oldPosX, const float* __restrict__ oldPosY,
+    const float* __restrict__ oldPosZ, const float* __restrict__ mass, float4* __restrict__ newBodyInfo,
+    const float4* __restrict__ oldVel, const float* __restrict__ oldVelX, const float* __restrict__ oldVelY,
+    const float* __restrict__ oldVelZ, float4* __restrict__ newVel, float damping, float softeningSqr, int n)
+{
+    __shared__ float4 tile[WORK_GROUP_SIZE_Y][ATOM_TILE];
+#if WORK_GROUP_SIZE_Y > 1
+    __shared__ float partial[WORK_GROUP_SIZE_Y - 1][3][BODIES_PER_THREAD][WORK_GROUP_SIZE_X];
+#endif
+
+    const int tx = threadIdx.x;
+    const int ty = threadIdx.y;
+    const int base = blockIdx.x * (WORK_GROUP_SIZE_X * BODIES_PER_THREAD);
+
+    float px[BODIES_PER_THREAD], py[BODIES_PER_THREAD], pz[BODIES_PER_THREAD];
+    float ax[BODIES_PER_THREAD], ay[BODIES_PER_THREAD], az[BODIES_PER_THREAD];
+#pragma unroll
+    for (int b = 0; b < BODIES_PER_THREAD; ++b)
+    {
+        const int i = base + b * WORK_GROUP_SIZE_X + tx;
+#if USE_SOA
+        px[b] = oldPosX[i];
+        py[b] = oldPosY[i];
+        pz[b] = oldPosZ[i];
+#else
+        const float4 p = oldBodyInfo[i];
+        px[b] = p.x;
+        py[b] = p.y;
+        pz[b] = p.z;
+#endif
+        ax[b] = 0.0f;
+        ay[b] = 0.0f;
+        az[b] = 0.0f;
+    }
+
+    const int rounds = n / (ATOM_TILE * WORK_GROUP_SIZE_Y);
+    for (int round = 0; round < rounds; ++round)
+    {
+        const int first = (round * WORK_GROUP_SIZE_Y + ty) * ATOM_TILE;
+        for (int k = tx; k < ATOM_TILE; k += WORK_GROUP_SIZE_X)
+        {
+#if USE_SOA
+            tile[ty][k] = make_float4(oldPosX[first + k], oldPosY[first + k], oldPosZ[first + k], mass[first + k]);
+#else
+            tile[ty][k] = oldBodyInfo[first + k];
+#endif
+        }
+        __syncthreads();
+#pragma unroll INNER_UNROLL_FACTOR
+        for (int j = 0; j < ATOM_TILE; ++j)
+        {
+            const float4 q = tile[ty][j];
+#pragma unroll
+            for (int b = 0; b < BODIES_PER_THREAD; ++b)
+            {
+                const float dx = q.x - px[b];
+                const float dy = q.y - py[b];
+                const float dz = q.z - pz[b];
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, softeningSqr)));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float s = q.w * inv * (inv * inv);
+                ax[b] = fmaf(s, dx, ax[b]);
+                ay[b] = fmaf(s, dy, ay[b]);
+                az[b] = fmaf(s, dz, az[b]);
+            }
+        }
+        __syncthreads();
+    }
+
+#if WORK_GROUP_SIZE_Y > 1
+    if (ty > 0)
+    {
+#pragma unroll
+        for (int b = 0; b < BODIES_PER_THREAD; ++b)
+        {
+            partial[ty - 1][0][b][tx] = ax[b];
+            partial[ty - 1][1][b][tx] = ay[b];
+            partial[ty - 1][2][b][tx] = az[b];
+        }
+    }
+    __syncthreads();
+    if (ty > 0)
+    {
+        return;
+    }
+    for (int row = 0; row < WORK_GROUP_SIZE_Y - 1; ++row)
+    {
+#pragma unroll
+        for (int b = 0; b < BODIES_PER_THREAD; ++b)
+        {
+            ax[b] += partial[row][0][b][tx];
+            ay[b] += partial[row][1][b][tx];
+            az[b] += partial[row][2][b][tx];
+        }
+    }
+#endif
+
+    const float dtdt = damping * timeDelta * timeDelta;
+#pragma unroll
+    for (int b = 0; b < BODIES_PER_THREAD; ++b)
+    {
+        const int i = base + b * WORK_GROUP_SIZE_X + tx;
+#if USE_SOA
+        const float vx = oldVelX[i];
+        const float vy = oldVelY[i];
+        const float vz = oldVelZ[i];
+        const float m = mass[i];
+#else
+        const float4 v = oldVel[i];
+        const float vx = v.x;
+        const float vy = v.y;
+        const float vz = v.z;
+        const float m = oldBodyInfo[i].w;
+#endif
+        newBodyInfo[i] = make_float4(px[b] + (timeDelta * vx + dtdt * ax[b]), py[b] + (timeDelta * vy + dtdt * ay[b]),
+            pz[b] + (timeDelta * vz + dtdt * az[b]), m);
+        newVel[i] = make_float4(vx + timeDelta * ax[b], vy + timeDelta * ay[b], vz + timeDelta * az[b], 0.0f);
+    }
+}
