@@ -61,7 +61,7 @@ KERNEL_CHECKS = [
     ("CoulombBatched.hip", "coulombBatchedReset", None),
 ]
 
-EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative", "KttTuningLauncher", "ProfiledCoulombSum3d"]
+EXAMPLES = ["CoulombSum2d", "CoulombSum3d", "CoulombSum3dIterative", "KttTuningLauncher", "ProfiledCoulombSum3d", "Nbody"]
 
 
 def _run(command: list[str]) -> None:
