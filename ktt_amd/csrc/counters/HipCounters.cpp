@@ -153,11 +153,12 @@ void CollectHardware(const State& state, const std::string& name, std::set<std::
     visiting.erase(name);
 }
 
-// Whether `name` or a counter it is derived from uses accumulate().
-bool ExpressionAccumulates(const State& state, const std::string& name)
+// Whether `name` or a counter it is derived from uses accumulate(). Definitions nest a few
+// levels at most; the depth bound also ends a cyclic definition.
+bool ExpressionAccumulates(const State& state, const std::string& name, const int depth = 0)
 {
     const auto found = state.counters.find(name);
-    if (found == state.counters.end() || !found->second.derived)
+    if (depth > 16 || found == state.counters.end() || !found->second.derived)
     {
         return false;
     }
@@ -175,7 +176,7 @@ bool ExpressionAccumulates(const State& state, const std::string& name)
             const bool startOk = at == 0 || !(std::isalnum(static_cast<unsigned char>(expression[at - 1])) || expression[at - 1] == '_');
             const size_t end = at + other.size();
             const bool endOk = end == expression.size() || !(std::isalnum(static_cast<unsigned char>(expression[end])) || expression[end] == '_');
-            if (startOk && endOk && ExpressionAccumulates(state, other))
+            if (startOk && endOk && ExpressionAccumulates(state, other, depth + 1))
             {
                 return true;
             }
