@@ -341,17 +341,18 @@ def simulate_search(results_file: str, searcher: str, knowledge_file: str | None
 
 def nbody_step(bodies: DeviceBuffer, velocity: DeviceBuffer, n: int, dt: float, damping: float, softening_sqr: float,
                new_bodies: DeviceBuffer, new_velocity: DeviceBuffer, config=None, stream=None, timed: bool = False):
-    ms = _timing(timed)
-    _check(library().ktt_nbody_step(bodies.pointer, velocity.pointer, n, dt, damping, softening_sqr, new_bodies.pointer,
-                                    new_velocity.pointer, _config(config), stream, ms), "ktt_nbody_step")
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_nbody_step(bodies.ptr, velocity.ptr, n, dt, damping, softening_sqr, new_bodies.ptr, new_velocity.ptr,
+                                    _config(config), stream, ctypes.byref(ms) if timed else None), "ktt_nbody_step")
     return ms.value if timed else None
 
 
 def nbody_step_reference(bodies: DeviceBuffer, velocity: DeviceBuffer, n: int, dt: float, damping: float, softening_sqr: float,
                          new_bodies: DeviceBuffer, new_velocity: DeviceBuffer, stream=None, timed: bool = False):
-    ms = _timing(timed)
-    _check(library().ktt_nbody_step_reference(bodies.pointer, velocity.pointer, n, dt, damping, softening_sqr, new_bodies.pointer,
-                                              new_velocity.pointer, stream, ms), "ktt_nbody_step_reference")
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_nbody_step_reference(bodies.ptr, velocity.ptr, n, dt, damping, softening_sqr, new_bodies.ptr,
+                                              new_velocity.ptr, stream, ctypes.byref(ms) if timed else None),
+           "ktt_nbody_step_reference")
     return ms.value if timed else None
 
 

@@ -5,9 +5,12 @@
 // Usage: ProfiledCoulombSum3d counters [grid side] [atoms] [counter,counter,...]
 //            profiles one configuration, repeating the launch until every counter pass ran,
 //            twice, and prints the counters as JSON;
-//        ProfiledCoulombSum3d search [max configurations] [grid side] [atoms] [results file]
+//        ProfiledCoulombSum3d search [max configurations, 0 = all] [grid side] [atoms] [results file]
 //            [knowledge base results file]
-//            tunes with the profile-based searcher and saves the results (with counters).
+//            tunes with the profile-based searcher and saves the results (with counters);
+//        ProfiledCoulombSum3d profile-all [max configurations, 0 = all] [grid side] [atoms] [results file]
+//            runs every configuration in index order with profiling on (every result with counters).
+//        A seventh argument "reduced" limits both to the shapes with INNER_UNROLL_FACTOR 1 and USE_SOA 0.
 // Counters are enabled before the tuner initialises the HIP runtime.
 #include <cstdio>
 #include <memory>
@@ -76,7 +79,7 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "hardware counters unavailable: %s\n", error.c_str());
         return 3;
     }
-    const bool search = mode == "search";
+    const bool search = mode == "search" || mode == "profile-all";
     const uint64_t maxConfigurations = search && argc > 2 ? std::stoull(argv[2]) : 30;
     const int base = search ? 3 : 2;
     const int gridSize = argc > base ? std::stoi(argv[base]) : 128;
@@ -106,6 +109,11 @@ int main(int argc, char** argv)
     const auto gridArgument = tuner.AddArgumentVector(p.grid, ktt::ArgumentAccessType::ReadWrite);
     ktt::AddCoulomb3dTuningSpace(tuner, kernel, definition);
     tuner.AddParameter(kernel, "USE_SOA", std::vector<uint64_t>{0, 1});
+    if (search && argc > 7 && std::string(argv[7]) == "reduced")
+    {
+        // The 288 shapes with INNER_UNROLL_FACTOR 1 and USE_SOA 0 (of 1,728).
+        tuner.AddConstraint(kernel, {"INNER_UNROLL_FACTOR", "USE_SOA"}, [](const std::vector<uint64_t>& v) { return v[0] == 1 && v[1] == 0; });
+    }
     tuner.SetArguments(definition, {ai, ax, ay, az, aw, n, h, g, gridArgument});
     tuner.SetArguments(referenceDefinition, {ai, n, h, g, gridArgument});
     tuner.SetReferenceKernel(gridArgument, referenceKernel, ktt::KernelConfiguration());
@@ -167,8 +175,23 @@ int main(int argc, char** argv)
 
     const std::string output = argc > 5 ? argv[5] : "ProfiledCoulombSum3dOutput";
     const std::string knowledge = argc > 6 ? argv[6] : "";
-    tuner.SetProfileBasedSearcher(kernel, knowledge, true, 5, 100, 10);
-    const auto results = tuner.Tune(kernel, std::make_unique<ktt::ConfigurationCount>(maxConfigurations));
+    if (mode == "profile-all")
+    {
+        // Every configuration in index order, each profiled: a knowledge base for the
+        // profile-based searcher, or saved results to replay it on.
+        tuner.SetSearcher(kernel, std::make_unique<ktt::DeterministicSearcher>());
+        tuner.SetProfiling(true);
+    }
+    else
+    {
+        tuner.SetProfileBasedSearcher(kernel, knowledge, true, 5, 100, 10);
+    }
+    std::unique_ptr<ktt::StopCondition> stop;
+    if (maxConfigurations > 0)
+    {
+        stop = std::make_unique<ktt::ConfigurationCount>(maxConfigurations);
+    }
+    const auto results = tuner.Tune(kernel, std::move(stop));
     tuner.SaveResults(results, output, ktt::OutputFormat::JSON);
     size_t valid = 0;
     size_t profiled = 0;
@@ -188,7 +211,7 @@ int main(int argc, char** argv)
             ++profiled;
         }
     }
-    std::printf("{\"mode\": \"search\", \"tested\": %zu, \"valid\": %zu, \"profiled\": %zu, \"best_us\": %.3f, \"space\": %llu}\n",
-        results.size(), valid, profiled, bestUs, static_cast<unsigned long long>(tuner.GetConfigurationsCount(kernel)));
+    std::printf("{\"mode\": \"%s\", \"tested\": %zu, \"valid\": %zu, \"profiled\": %zu, \"best_us\": %.3f, \"space\": %llu}\n",
+        mode.c_str(), results.size(), valid, profiled, bestUs, static_cast<unsigned long long>(tuner.GetConfigurationsCount(kernel)));
     return valid > 0 ? 0 : 1;
 }

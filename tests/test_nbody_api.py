@@ -35,3 +35,18 @@ def test_nbody_step_rejects(n, config, bodies, message):
 def test_nbody_reference_needs_whole_workgroups():
     status = capi.library().ktt_nbody_step_reference(FAKE, FAKE, 100, 0.001, 0.5, 0.01, FAKE, FAKE, None, None)
     assert status == -1 and b"multiple of 64" in capi.library().ktt_hip_last_error()
+
+
+class _Unallocated:
+    """Stands in for a DeviceBuffer; the calls below fail their checks before using it."""
+
+    ptr = FAKE
+
+
+def test_python_wrappers_marshal_their_arguments():
+    for wrapper, message in ((capi.nbody_step, "must be a multiple"), (capi.nbody_step_reference, "multiple of 64")):
+        with pytest.raises(capi.KttHipError, match=message):
+            wrapper(_Unallocated, _Unallocated, 1000, 0.001, 0.5, 0.01, _Unallocated, _Unallocated, timed=True)
+    import numpy as np
+    with pytest.raises(capi.KttHipError, match="multiple of 64"):
+        capi.nbody_tune(np.zeros((100, 4), np.float32), np.zeros((100, 4), np.float32), 0.001, 0.5, 0.01)
