@@ -473,7 +473,8 @@ void ProfileBasedSearcher::FillRandomBatch(const size_t count)
     {
         chosen.insert(GetIndex(configuration));
     }
-    const size_t target = std::min<uint64_t>(m_Batch.size() + count, m_Batch.size() + GetUnexploredConfigurationsCount());
+    // Batch members are unexplored themselves, so the unexplored count bounds the batch.
+    const size_t target = std::max<uint64_t>(m_Batch.size(), std::min<uint64_t>(m_Batch.size() + count, GetUnexploredConfigurationsCount()));
     while (m_Batch.size() < target)
     {
         const KernelConfiguration drawn = GetRandomConfiguration();
