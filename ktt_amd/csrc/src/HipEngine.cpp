@@ -1,7 +1,14 @@
 #include <hip/hiprtc.h>
 
+#include <unistd.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iterator>
 #include <sstream>
 
 #include <KttException.h>
@@ -67,7 +74,139 @@ std::vector<std::string> HipCompiler::DefaultOptions(const std::string& architec
     return {"--offload-arch=" + architecture, "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
 }
 
+// ---------------------------------------------------------------- code-object cache
+//
+// With KTT_HIP_CACHE_DIR set, compiled code objects are kept on disk: <key>.co holds the code
+// object and <key>.name the kernel's lowered name. The key hashes the source, the kernel name,
+// the options, the hipRTC version and a fingerprint (name, size, modification time) of every
+// file in the options' -I directories, so editing an included header invalidates the entry.
+// Files are written under a temporary name and renamed, so concurrent writers never expose a
+// partial entry.
+
+namespace
+{
+
+uint64_t Fnv1a(const std::string& text, uint64_t hash)
+{
+    for (const unsigned char c : text)
+    {
+        hash ^= c;
+        hash *= 1099511628211ull;
+    }
+    return hash;
+}
+
+std::string IncludeFingerprint(const std::vector<std::string>& options)
+{
+    std::string fingerprint;
+    for (const auto& option : options)
+    {
+        if (option.rfind("-I", 0) != 0 || option.size() <= 2)
+        {
+            continue;
+        }
+        std::error_code error;
+        std::vector<std::string> entries;
+        for (const auto& entry : std::filesystem::directory_iterator(option.substr(2), error))
+        {
+            if (!entry.is_regular_file(error))
+            {
+                continue;
+            }
+            const auto time = std::filesystem::last_write_time(entry.path(), error).time_since_epoch().count();
+            entries.push_back(entry.path().filename().string() + ":" + std::to_string(entry.file_size(error)) + ":"
+                + std::to_string(static_cast<long long>(time)));
+        }
+        std::sort(entries.begin(), entries.end());
+        for (const auto& entry : entries)
+        {
+            fingerprint += entry + ";";
+        }
+    }
+    return fingerprint;
+}
+
+std::string CacheKey(const std::string& source, const std::string& kernelName, const std::vector<std::string>& options)
+{
+    int major = 0;
+    int minor = 0;
+    hiprtcVersion(&major, &minor);
+    std::string material = source + '\0' + kernelName + '\0';
+    for (const auto& option : options)
+    {
+        material += option + '\0';
+    }
+    material += std::to_string(major) + "." + std::to_string(minor) + '\0' + IncludeFingerprint(options);
+    char key[40];
+    std::snprintf(key, sizeof(key), "%016llx%016llx", static_cast<unsigned long long>(Fnv1a(material, 1469598103934665603ull)),
+        static_cast<unsigned long long>(Fnv1a(material, 0x9E3779B97F4A7C15ull)));
+    return key;
+}
+
+bool ReadFile(const std::string& path, std::string& contents)
+{
+    std::ifstream stream(path, std::ios::binary);
+    if (!stream)
+    {
+        return false;
+    }
+    contents.assign(std::istreambuf_iterator<char>(stream), std::istreambuf_iterator<char>());
+    return true;
+}
+
+void WriteFileAtomically(const std::string& path, const char* data, const size_t size)
+{
+    const std::string temporary = path + ".tmp" + std::to_string(static_cast<long long>(getpid()));
+    {
+        std::ofstream stream(temporary, std::ios::binary | std::ios::trunc);
+        if (!stream)
+        {
+            return;
+        }
+        stream.write(data, static_cast<std::streamsize>(size));
+        if (!stream)
+        {
+            std::remove(temporary.c_str());
+            return;
+        }
+    }
+    std::rename(temporary.c_str(), path.c_str());
+}
+
+} // namespace
+
 std::vector<char> HipCompiler::Compile(const std::string& source, const std::string& programName, const std::string& kernelName,
+    const std::vector<std::string>& options, std::string& loweredName, std::string* log)
+{
+    const char* cacheDirectory = std::getenv("KTT_HIP_CACHE_DIR");
+    std::string cachePath;
+    if (cacheDirectory != nullptr && cacheDirectory[0] != '\0')
+    {
+        std::error_code error;
+        std::filesystem::create_directories(cacheDirectory, error);
+        cachePath = std::string(cacheDirectory) + "/" + CacheKey(source, kernelName, options);
+        std::string code;
+        std::string name;
+        if (ReadFile(cachePath + ".name", name) && ReadFile(cachePath + ".co", code) && !code.empty() && !name.empty())
+        {
+            loweredName = name;
+            if (log != nullptr)
+            {
+                log->clear();
+            }
+            return std::vector<char>(code.begin(), code.end());
+        }
+    }
+    std::vector<char> code = CompileWithRtc(source, programName, kernelName, options, loweredName, log);
+    if (!cachePath.empty())
+    {
+        WriteFileAtomically(cachePath + ".co", code.data(), code.size());
+        WriteFileAtomically(cachePath + ".name", loweredName.data(), loweredName.size());
+    }
+    return code;
+}
+
+std::vector<char> HipCompiler::CompileWithRtc(const std::string& source, const std::string& programName, const std::string& kernelName,
     const std::vector<std::string>& options, std::string& loweredName, std::string* log)
 {
     hiprtcProgram program;

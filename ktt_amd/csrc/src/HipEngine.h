@@ -66,8 +66,12 @@ private:
 class HipCompiler
 {
 public:
-    /** Compiles source for the given options; returns the code object and sets loweredName. Throws CompilerError. */
+    /** Compiles source for the given options; returns the code object and sets loweredName. Throws CompilerError.
+     *  With KTT_HIP_CACHE_DIR set, code objects are cached on disk under that directory. */
     static std::vector<char> Compile(const std::string& source, const std::string& programName, const std::string& kernelName,
+        const std::vector<std::string>& options, std::string& loweredName, std::string* log = nullptr);
+    /** The hipRTC compilation itself, without the cache. */
+    static std::vector<char> CompileWithRtc(const std::string& source, const std::string& programName, const std::string& kernelName,
         const std::vector<std::string>& options, std::string& loweredName, std::string* log = nullptr);
     /** Default options for the current device (or gfx950 when none is visible). */
     static std::vector<std::string> DefaultOptions(const std::string& architecture);
