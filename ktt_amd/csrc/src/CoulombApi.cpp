@@ -337,10 +337,18 @@ void CoulombRuntime::Launch(const HipKernel& kernel, const unsigned int grid[3],
 // thread, the compensated form (three values per point) 4 x 8.
 static CoulombConfig Defaults3d(const int form = KTT_COULOMB_PLAIN)
 {
+    // The fastest shapes of each form at 256^3 x 65,536 on MI355X (sweeps of the tuning space).
     CoulombConfig config;
-    const bool compensated = form == KTT_COULOMB_COMPENSATED;
-    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", compensated ? "4" : "2"},
-        {"ROWS_PER_THREAD", compensated ? "8" : "16"}, {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
+    if (form == KTT_COULOMB_COMPENSATED)
+    {
+        config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "2"}, {"ROWS_PER_THREAD", "4"},
+            {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "2"}};
+    }
+    else
+    {
+        config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "2"}, {"ROWS_PER_THREAD", "16"},
+            {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
+    }
     return config;
 }
 
