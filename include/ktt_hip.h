@@ -248,8 +248,9 @@ int ktt_hip_enable_counters(void);
  * nbody_kernel): a_i = sum_j m_j d_ij / (|d_ij|^2 + softening_sqr)^(3/2),
  * p' = p + dt v + damping dt^2 a, v' = v + dt a. body_info and new_body_info are n float4
  * (x, y, z, mass); velocity and new_velocity n float4 (vx, vy, vz, 0). config as in section 3
- * (WORK_GROUP_SIZE_X/Y, BODIES_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR; NULL: defaults); n
- * must be a multiple of WORK_GROUP_SIZE_X * BODIES_PER_THREAD and ATOM_TILE * WORK_GROUP_SIZE_Y. */
+ * (WORK_GROUP_SIZE_X/Y, BODIES_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR; NULL: 64, 4, 4, 256, 8); n
+ * must be a multiple of WORK_GROUP_SIZE_X * BODIES_PER_THREAD and ATOM_TILE * WORK_GROUP_SIZE_Y
+ * (1,024 for the defaults). */
 int ktt_nbody_step(const float* body_info, const float* velocity, int n, float time_delta, float damping,
                    float softening_sqr, float* new_body_info, float* new_velocity, const char* config, void* stream,
                    float* kernel_ms);

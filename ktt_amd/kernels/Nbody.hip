@@ -23,19 +23,19 @@
 #endif
 
 #ifndef WORK_GROUP_SIZE_X
-#define WORK_GROUP_SIZE_X 256
+#define WORK_GROUP_SIZE_X 64
 #endif
 #ifndef WORK_GROUP_SIZE_Y
-#define WORK_GROUP_SIZE_Y 1
+#define WORK_GROUP_SIZE_Y 4
 #endif
 #ifndef BODIES_PER_THREAD
-#define BODIES_PER_THREAD 2
+#define BODIES_PER_THREAD 4
 #endif
 #ifndef ATOM_TILE
 #define ATOM_TILE 256
 #endif
 #ifndef INNER_UNROLL_FACTOR
-#define INNER_UNROLL_FACTOR 4
+#define INNER_UNROLL_FACTOR 8
 #endif
 #ifndef USE_SOA
 #define USE_SOA 0
