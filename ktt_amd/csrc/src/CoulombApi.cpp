@@ -335,21 +335,34 @@ void CoulombRuntime::Launch(const HipKernel& kernel, const unsigned int grid[3],
 
 // Tuned defaults (gfx950, 256^3 grid x 65,536 atoms): the plain form keeps 2 x 16 points per
 // thread, the compensated form (three values per point) 4 x 8.
-static CoulombConfig Defaults3d(const int form = KTT_COULOMB_PLAIN)
+static CoulombConfig Defaults3d(const int form, const long gridSize)
 {
-    // The fastest shapes of each form at 256^3 x 65,536 on MI355X (sweeps of the tuning space).
-    // For the compensated form, splitting the atoms over 2 blocks is 1.3% faster still, but each
-    // block then writes and reads back FP64 partials of the whole grid (5x the HBM traffic).
+    // The fastest shapes of each form at 256^3 x 65,536 on MI355X (sweeps of workgroup shape,
+    // register block, atom tile and unroll).  The compensated form forms its row brackets in FP64,
+    // once per workgroup and x span, so it prefers a wide span (2 x 128 points); the plain form
+    // prefers one point per thread and 8 rows.  For the compensated form, splitting the atoms over
+    // 2 blocks is faster by about 1%, but each block then writes and reads back FP64 partials of
+    // the whole grid (5x the HBM traffic).  A row length that is not a multiple of the span
+    // narrows the workgroup to 64 threads in x, so that fewer lanes idle past the row's end.
     CoulombConfig config;
+    long wgx = 128;
     if (form == KTT_COULOMB_COMPENSATED)
     {
-        config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "2"}, {"ROWS_PER_THREAD", "4"},
-            {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
+        if (gridSize % 256 != 0)
+        {
+            wgx = 64;
+        }
+        config.values = {{"WORK_GROUP_SIZE_X", std::to_string(wgx)}, {"WORK_GROUP_SIZE_Y", "2"}, {"POINTS_X", "2"},
+            {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "256"}, {"INNER_UNROLL_FACTOR", "8"}, {"ATOM_BLOCKS", "1"}};
     }
     else
     {
-        config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "2"}, {"ROWS_PER_THREAD", "16"},
-            {"ATOM_TILE", "64"}, {"INNER_UNROLL_FACTOR", "1"}, {"ATOM_BLOCKS", "1"}};
+        if (gridSize % 128 != 0)
+        {
+            wgx = 64;
+        }
+        config.values = {{"WORK_GROUP_SIZE_X", std::to_string(wgx)}, {"WORK_GROUP_SIZE_Y", std::to_string(512 / wgx)},
+            {"POINTS_X", "1"}, {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "8"}, {"ATOM_BLOCKS", "1"}};
     }
     return config;
 }
@@ -664,7 +677,7 @@ int ktt_coulomb_launch_shape(int kind, int gridSize, const char* config, unsigne
     {
         return SetLastError("invalid arguments");
     }
-    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d() : DefaultsSlice());
+    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d(KTT_COULOMB_PLAIN, gridSize) : DefaultsSlice());
     const long rows = kind == 3 ? static_cast<long>(gridSize) * gridSize : gridSize;
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, rows, parsed.Get("ATOM_BLOCKS", 1));
     for (int i = 0; i < 3; ++i)
@@ -691,7 +704,7 @@ int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize
         return SetLastError("unknown form " + std::to_string(form));
     }
     auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, Defaults3d(form));
+    const CoulombConfig parsed = CoulombConfig::Parse(config, Defaults3d(form, gridSize));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks);
     const std::vector<std::string> defines = parsed.Defines(kLaunchOnly);

@@ -21,12 +21,12 @@ namespace ktt
  *  size (G, G*G, 1) and local size (1, 1, 1). */
 void AddCoulomb3dTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefinitionId definition)
 {
-    tuner.AddParameter(kernel, "WORK_GROUP_SIZE_X", std::vector<uint64_t>{16, 32, 64});
+    tuner.AddParameter(kernel, "WORK_GROUP_SIZE_X", std::vector<uint64_t>{16, 32, 64, 128, 256});
     tuner.AddParameter(kernel, "WORK_GROUP_SIZE_Y", std::vector<uint64_t>{1, 2, 4, 8});
     tuner.AddParameter(kernel, "POINTS_X", std::vector<uint64_t>{1, 2, 4, 8});
     tuner.AddParameter(kernel, "ROWS_PER_THREAD", std::vector<uint64_t>{4, 8, 16});
-    tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{32, 64, 128});
-    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1, 2, 4});
+    tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{32, 64, 128, 256});
+    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1, 2, 4, 8});
 
     tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::X, "WORK_GROUP_SIZE_X",
         ModifierAction::Multiply);

@@ -59,6 +59,8 @@ CONFIGS_3D = [
     "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=2,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=32",
     "WORK_GROUP_SIZE_X=16,WORK_GROUP_SIZE_Y=8,POINTS_X=8,ROWS_PER_THREAD=8,ATOM_TILE=128,INNER_UNROLL_FACTOR=4",
     "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=1,POINTS_X=2,ROWS_PER_THREAD=16,ATOM_TILE=64,ATOM_BLOCKS=3",
+    # The default shape for rows that are a multiple of 128 points, here over a 40-point row.
+    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=8,ATOM_TILE=128,INNER_UNROLL_FACTOR=8",
 ]
 
 
@@ -74,7 +76,12 @@ def test_plain_3d_matches_fp64_oracle(capi, config):
     assert np.max(np.abs(result - cpu)) <= SIDE_BY_SIDE
 
 
-@pytest.mark.parametrize("config", [None, "ATOM_BLOCKS=4", "POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=32,ATOM_BLOCKS=7"])
+@pytest.mark.parametrize("config", [None, "ATOM_BLOCKS=4", "POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=32,ATOM_BLOCKS=7",
+                                    # the default shape for rows that are a multiple of 256 points
+                                    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=256,"
+                                    "INNER_UNROLL_FACTOR=8",
+                                    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=256,"
+                                    "INNER_UNROLL_FACTOR=8,ATOM_BLOCKS=3"])
 def test_compensated_3d_matches_fp64_oracle(capi, config):
     atoms = workloads.uniform_atoms(22, 1000, box=20.0)
     grid, spacing = 40, 0.5
