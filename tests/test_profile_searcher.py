@@ -16,30 +16,11 @@ import numpy as np
 import pytest
 
 from ktt_amd import capi
+from profiled_tables import expand
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 SEEDS = range(1, 101)
 SM_COUNT = 68  # RTX 2080 Ti
-
-
-def expand(table: dict) -> dict:
-    """KTT JSON results of the compact counter table."""
-    count = len(table["Parameters"])
-    results = []
-    for row in table["Rows"]:
-        configuration = [{"Name": n, "Value": v, "ValueType": t} for n, v, t in zip(table["Parameters"], row[:count], table["ValueTypes"])]
-        status, total, duration = row[count:count + 3]
-        global_size, local_size, values = row[count + 3:count + 6], row[count + 6:count + 9], row[count + 9:]
-        computation = {
-            "KernelFunction": table["KernelFunction"], "Duration": duration, "Overhead": 0.0,
-            "GlobalSize": dict(zip("XYZ", global_size)), "LocalSize": dict(zip("XYZ", local_size)),
-            "ProfilingData": {"RemainingProfilingRuns": 0,
-                              "Counters": [{"Name": n, "Type": "Double", "Value": v} for n, v in zip(table["Counters"], values)]},
-        }
-        results.append({"KernelName": table["KernelName"], "Status": status, "TotalDuration": total, "TotalOverhead": 0.0,
-                        "ExtraDuration": max(0.0, total - duration), "DataMovementOverhead": 0.0, "ValidationOverhead": 0.0,
-                        "SearcherOverhead": 0.0, "Configuration": configuration, "ComputationResults": [computation]})
-    return {"Metadata": table["Metadata"], "Results": results}
 
 
 @pytest.fixture(scope="module")
