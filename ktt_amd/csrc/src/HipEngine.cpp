@@ -68,6 +68,21 @@ std::string HipCompiler::DeviceArchitecture(const int device)
     return name.empty() ? "gfx950" : name;
 }
 
+std::string HipCompiler::DeviceName(const int device)
+{
+    hipDeviceProp_t properties;
+    if (hipGetDeviceProperties(&properties, device) == hipSuccess && properties.name[0] != '\0')
+    {
+        return properties.name;
+    }
+    char name[256] = {};
+    if (hipDeviceGetName(name, static_cast<int>(sizeof(name)), device) == hipSuccess && name[0] != '\0')
+    {
+        return name;
+    }
+    return "AMD GPU (" + DeviceArchitecture(device) + ")";
+}
+
 std::vector<std::string> HipCompiler::DefaultOptions(const std::string& architecture)
 {
     // Packed FP32 instructions are disabled for every kernel this library builds.
@@ -389,7 +404,7 @@ void HipEngine::Initialize()
     CheckHipError(hipSetDevice(m_DeviceIndex), "hipSetDevice");
     CheckHipError(hipGetDeviceProperties(&m_Properties, m_DeviceIndex), "hipGetDeviceProperties");
     m_Architecture = HipCompiler::DeviceArchitecture(m_DeviceIndex);
-    Logger::LogInfo("Initializing HIP engine on device " + std::string(m_Properties.name) + " (" + m_Architecture + ")");
+    Logger::LogInfo("Initializing HIP engine on device " + HipCompiler::DeviceName(m_DeviceIndex) + " (" + m_Architecture + ")");
 }
 
 HipEngine::~HipEngine()
@@ -1223,7 +1238,7 @@ std::vector<DeviceInfo> HipEngine::GetDeviceInfo(const PlatformIndex platformInd
     {
         hipDeviceProp_t properties;
         CheckHipError(hipGetDeviceProperties(&properties, device), "hipGetDeviceProperties");
-        DeviceInfo info(static_cast<DeviceIndex>(device), properties.name);
+        DeviceInfo info(static_cast<DeviceIndex>(device), HipCompiler::DeviceName(device));
         info.SetVendor("Advanced Micro Devices, Inc.");
         info.SetArchitecture(HipCompiler::DeviceArchitecture(device));
         info.SetGlobalMemorySize(properties.totalGlobalMem);

@@ -77,6 +77,9 @@ public:
     static std::vector<std::string> DefaultOptions(const std::string& architecture);
     /** Architecture of the device without feature suffixes, e.g. "gfx950". */
     static std::string DeviceArchitecture(const int device);
+    /** Marketing name of the device; when the runtime reports none (some containers hide it),
+     *  hipDeviceGetName's, else "AMD GPU (<architecture>)". */
+    static std::string DeviceName(const int device);
 };
 
 class HipEngine : public ComputeEngine

@@ -52,6 +52,7 @@ def test_device_is_gfx950(capi):
     print("device:", info, flush=True)
     assert info["arch"] == "gfx950"
     assert info["compute_units"] >= 1
+    assert info["name"], "the device name falls back to hipDeviceGetName, then to the architecture"
 
 
 CONFIGS_3D = [
