@@ -47,13 +47,17 @@ struct BucketShape
     int px;
     int rows;
     int tile;
+    int unroll = 1;  // unroll of the per-tile atom loop
 
     int Span() const { return wgx * px; }
     int RowsPerGroup() const { return (kBatchThreads / wgx) * rows; }
 };
 
-// Defaults for buckets 0..4 (the kernel's own defaults; the host always passes them explicitly).
-const BucketShape kDefaultShapes[kMaxBuckets] = {{16, 2, 4, 64}, {16, 3, 4, 64}, {32, 2, 8, 64}, {32, 3, 8, 64}, {32, 4, 8, 64}};
+// Defaults for buckets 0..4 (the kernel's own defaults; the host always passes them explicitly):
+// the fastest shapes of a bucket tuning run (MCMC) on the 1,024-system batch of grid sides 32, 48,
+// 64, 96 and 128 on MI355X, with the atom loop unrolled 8 times in every bucket.
+const BucketShape kDefaultShapes[kMaxBuckets] = {{16, 2, 4, 64, 8}, {16, 3, 4, 64, 8}, {32, 2, 8, 64, 8}, {32, 3, 8, 128, 8},
+    {64, 2, 8, 128, 8}};
 
 // Plain single-system kernel configuration of the per-system baseline, by grid side.
 std::string PlainConfigForSide(const int side)
@@ -68,9 +72,10 @@ std::string PlainConfigForSide(const int side)
 void CheckShape(const BucketShape& shape, const int bucket)
 {
     const std::string name = "bucket " + std::to_string(bucket);
-    if (shape.wgx <= 0 || kBatchThreads % shape.wgx != 0 || shape.px <= 0 || shape.rows <= 0 || shape.tile <= 0)
+    if (shape.wgx <= 0 || kBatchThreads % shape.wgx != 0 || shape.px <= 0 || shape.rows <= 0 || shape.tile <= 0
+        || shape.unroll <= 0)
     {
-        throw KttException(name + ": WGX must divide " + std::to_string(kBatchThreads) + " and PX, ROWS, TILE must be positive");
+        throw KttException(name + ": WGX must divide " + std::to_string(kBatchThreads) + " and PX, ROWS, TILE, UNROLL must be positive");
     }
     const int rows = shape.RowsPerGroup();
     if (kBatchThreads % rows != 0 && rows % kBatchThreads != 0)
@@ -211,6 +216,7 @@ CoulombBatch::CoulombBatch(const float* hostAtoms, const int* atomCounts, const 
         shape.px = static_cast<int>(parsed.Get(prefix + "PX", shape.px));
         shape.rows = static_cast<int>(parsed.Get(prefix + "ROWS", shape.rows));
         shape.tile = static_cast<int>(parsed.Get(prefix + "TILE", shape.tile));
+        shape.unroll = static_cast<int>(parsed.Get(prefix + "UNROLL", shape.unroll));
         CheckShape(shape, b);
         m_Shapes.push_back(shape);
     }
@@ -378,6 +384,7 @@ void CoulombBatch::CompileKernels()
         defines.Set(prefix + "PX", std::to_string(m_Shapes[b].px));
         defines.Set(prefix + "ROWS", std::to_string(m_Shapes[b].rows));
         defines.Set(prefix + "TILE", std::to_string(m_Shapes[b].tile));
+        defines.Set(prefix + "UNROLL", std::to_string(m_Shapes[b].unroll));
     }
     defines.Set("BATCH_THREADS", std::to_string(kBatchThreads));
     auto& runtime = CoulombRuntime::Get();
@@ -405,7 +412,7 @@ std::string CoulombBatch::ShapeString() const
         const std::string prefix = "BUCKET" + std::to_string(b) + "_";
         result += (result.empty() ? "" : ",") + prefix + "WGX=" + std::to_string(m_Shapes[b].wgx) + "," + prefix + "PX="
             + std::to_string(m_Shapes[b].px) + "," + prefix + "ROWS=" + std::to_string(m_Shapes[b].rows) + "," + prefix + "TILE="
-            + std::to_string(m_Shapes[b].tile);
+            + std::to_string(m_Shapes[b].tile) + "," + prefix + "UNROLL=" + std::to_string(m_Shapes[b].unroll);
     }
     return result + ",GROUPS_PER_CU=" + std::to_string(m_GroupsPerCu);
 }
@@ -619,6 +626,7 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
     tuner.AddParameter(kernel, prefix + "PX", std::vector<uint64_t>{1, 2, 3, 4});
     tuner.AddParameter(kernel, prefix + "ROWS", std::vector<uint64_t>{2, 4, 8});
     tuner.AddParameter(kernel, prefix + "TILE", std::vector<uint64_t>{32, 64, 128});
+    tuner.AddParameter(kernel, prefix + "UNROLL", std::vector<uint64_t>{1, 8});
     // Every other bucket keeps its current shape (single-valued parameters are defines too).
     for (int b = 0; b < kMaxBuckets; ++b)
     {
@@ -631,6 +639,7 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
         tuner.AddParameter(kernel, other + "PX", std::vector<uint64_t>{static_cast<uint64_t>(batch.Shapes()[b].px)});
         tuner.AddParameter(kernel, other + "ROWS", std::vector<uint64_t>{static_cast<uint64_t>(batch.Shapes()[b].rows)});
         tuner.AddParameter(kernel, other + "TILE", std::vector<uint64_t>{static_cast<uint64_t>(batch.Shapes()[b].tile)});
+        tuner.AddParameter(kernel, other + "UNROLL", std::vector<uint64_t>{static_cast<uint64_t>(batch.Shapes()[b].unroll)});
     }
     const int side = batch.BucketSides()[static_cast<size_t>(bucket)];
     tuner.AddConstraint(kernel, {prefix + "WGX", prefix + "PX", prefix + "ROWS", prefix + "TILE"},
@@ -653,7 +662,7 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
         const std::string prefix = "BUCKET" + std::to_string(bucket) + "_";
         shapes[static_cast<size_t>(bucket)] = BucketShape{ParameterPair::GetParameterValue<int>(pairs, prefix + "WGX"),
             ParameterPair::GetParameterValue<int>(pairs, prefix + "PX"), ParameterPair::GetParameterValue<int>(pairs, prefix + "ROWS"),
-            ParameterPair::GetParameterValue<int>(pairs, prefix + "TILE")};
+            ParameterPair::GetParameterValue<int>(pairs, prefix + "TILE"), ParameterPair::GetParameterValue<int>(pairs, prefix + "UNROLL")};
         const auto items = batch.BuildItems(shapes);
         batch.UploadItems(items);
         int itemCount = static_cast<int>(items.size() / 4);
@@ -702,13 +711,13 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
     KernelResult incumbentResult;
     const ParameterInput incumbentInput = {{prefix + "WGX", static_cast<uint64_t>(incumbent.wgx)},
         {prefix + "PX", static_cast<uint64_t>(incumbent.px)}, {prefix + "ROWS", static_cast<uint64_t>(incumbent.rows)},
-        {prefix + "TILE", static_cast<uint64_t>(incumbent.tile)}};
+        {prefix + "TILE", static_cast<uint64_t>(incumbent.tile)}, {prefix + "UNROLL", static_cast<uint64_t>(incumbent.unroll)}};
     const auto inList = [](const int value, const std::initializer_list<int> list)
     {
         return std::find(list.begin(), list.end(), value) != list.end();
     };
     if (inList(incumbent.wgx, {16, 32, 64}) && inList(incumbent.px, {1, 2, 3, 4}) && inList(incumbent.rows, {2, 4, 8})
-        && inList(incumbent.tile, {32, 64, 128}))
+        && inList(incumbent.tile, {32, 64, 128}) && inList(incumbent.unroll, {1, 8}))
     {
         incumbentResult = tuner.Run(kernel, tuner.CreateConfiguration(kernel, incumbentInput), {});
         if (incumbentResult.IsValid() && (winner == nullptr || incumbentResult.GetKernelDuration() < winner->GetKernelDuration()))
@@ -722,7 +731,8 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
     }
     const auto& pairs = winner->GetConfiguration().GetPairs();
     best = BucketShape{ParameterPair::GetParameterValue<int>(pairs, prefix + "WGX"), ParameterPair::GetParameterValue<int>(pairs, prefix + "PX"),
-        ParameterPair::GetParameterValue<int>(pairs, prefix + "ROWS"), ParameterPair::GetParameterValue<int>(pairs, prefix + "TILE")};
+        ParameterPair::GetParameterValue<int>(pairs, prefix + "ROWS"), ParameterPair::GetParameterValue<int>(pairs, prefix + "TILE"),
+        ParameterPair::GetParameterValue<int>(pairs, prefix + "UNROLL")};
     bestMs = static_cast<double>(winner->GetKernelDuration()) / 1.0e6;
     return *winner;
 }
@@ -850,6 +860,7 @@ int ktt_coulomb_batch_tune(void* batch, int searcher, int maxConfigurationsPerBu
         entry["PX"] = best.px;
         entry["ROWS"] = best.rows;
         entry["TILE"] = best.tile;
+        entry["UNROLL"] = best.unroll;
         entry["kernel_ms"] = bestMs;
         entry["configurations_tested"] = tested;
         buckets.PushBack(entry);

@@ -12,8 +12,8 @@
 //
 // Systems are grouped into up to five tile-shape buckets (by grid side);
 // bucket b uses BUCKETb_WGX x (BATCH_THREADS / BUCKETb_WGX) threads,
-// BUCKETb_PX x BUCKETb_ROWS points per thread and BUCKETb_TILE atoms per LDS
-// tile, chosen per bucket by the autotuner.
+// BUCKETb_PX x BUCKETb_ROWS points per thread, BUCKETb_TILE atoms per LDS
+// tile and an unroll of BUCKETb_UNROLL atoms, chosen per bucket by the autotuner.
 #include "coulomb_core.h"
 
 #ifndef BATCH_THREADS
@@ -65,19 +65,34 @@
 #define BUCKET3_ROWS 8
 #endif
 #ifndef BUCKET3_TILE
-#define BUCKET3_TILE 64
+#define BUCKET3_TILE 128
 #endif
 #ifndef BUCKET4_WGX
-#define BUCKET4_WGX 32
+#define BUCKET4_WGX 64
 #endif
 #ifndef BUCKET4_PX
-#define BUCKET4_PX 4
+#define BUCKET4_PX 2
 #endif
 #ifndef BUCKET4_ROWS
 #define BUCKET4_ROWS 8
 #endif
 #ifndef BUCKET4_TILE
-#define BUCKET4_TILE 64
+#define BUCKET4_TILE 128
+#endif
+#ifndef BUCKET0_UNROLL
+#define BUCKET0_UNROLL 8
+#endif
+#ifndef BUCKET1_UNROLL
+#define BUCKET1_UNROLL 8
+#endif
+#ifndef BUCKET2_UNROLL
+#define BUCKET2_UNROLL 8
+#endif
+#ifndef BUCKET3_UNROLL
+#define BUCKET3_UNROLL 8
+#endif
+#ifndef BUCKET4_UNROLL
+#define BUCKET4_UNROLL 8
 #endif
 
 // One system of the batch (32 bytes; the host writes the same layout).
@@ -95,11 +110,11 @@ struct BatchSystem
 namespace
 {
 
-using E0 = kc::Engine<BUCKET0_WGX, BATCH_THREADS / BUCKET0_WGX, BUCKET0_PX, BUCKET0_ROWS, BUCKET0_TILE, 1>;
-using E1 = kc::Engine<BUCKET1_WGX, BATCH_THREADS / BUCKET1_WGX, BUCKET1_PX, BUCKET1_ROWS, BUCKET1_TILE, 1>;
-using E2 = kc::Engine<BUCKET2_WGX, BATCH_THREADS / BUCKET2_WGX, BUCKET2_PX, BUCKET2_ROWS, BUCKET2_TILE, 1>;
-using E3 = kc::Engine<BUCKET3_WGX, BATCH_THREADS / BUCKET3_WGX, BUCKET3_PX, BUCKET3_ROWS, BUCKET3_TILE, 1>;
-using E4 = kc::Engine<BUCKET4_WGX, BATCH_THREADS / BUCKET4_WGX, BUCKET4_PX, BUCKET4_ROWS, BUCKET4_TILE, 1>;
+using E0 = kc::Engine<BUCKET0_WGX, BATCH_THREADS / BUCKET0_WGX, BUCKET0_PX, BUCKET0_ROWS, BUCKET0_TILE, BUCKET0_UNROLL>;
+using E1 = kc::Engine<BUCKET1_WGX, BATCH_THREADS / BUCKET1_WGX, BUCKET1_PX, BUCKET1_ROWS, BUCKET1_TILE, BUCKET1_UNROLL>;
+using E2 = kc::Engine<BUCKET2_WGX, BATCH_THREADS / BUCKET2_WGX, BUCKET2_PX, BUCKET2_ROWS, BUCKET2_TILE, BUCKET2_UNROLL>;
+using E3 = kc::Engine<BUCKET3_WGX, BATCH_THREADS / BUCKET3_WGX, BUCKET3_PX, BUCKET3_ROWS, BUCKET3_TILE, BUCKET3_UNROLL>;
+using E4 = kc::Engine<BUCKET4_WGX, BATCH_THREADS / BUCKET4_WGX, BUCKET4_PX, BUCKET4_ROWS, BUCKET4_TILE, BUCKET4_UNROLL>;
 
 static_assert(E0::Threads == BATCH_THREADS && E1::Threads == BATCH_THREADS && E2::Threads == BATCH_THREADS
     && E3::Threads == BATCH_THREADS && E4::Threads == BATCH_THREADS, "every bucket shape must use BATCH_THREADS threads");

@@ -37,9 +37,15 @@ def check_against_oracle(systems, grids, bar, samples=300):
     return worst
 
 
-def test_batch_matches_oracle_and_graph_replays_match_eager(capi):
+UNROLLED = ("BUCKET0_UNROLL=8,BUCKET1_UNROLL=8,BUCKET2_UNROLL=8,BUCKET1_TILE=32,BUCKET2_WGX=64,BUCKET2_PX=1,BUCKET2_ROWS=4,"
+            "BUCKET2_TILE=128")
+
+
+@pytest.mark.parametrize("config", [None, UNROLLED])
+def test_batch_matches_oracle_and_graph_replays_match_eager(capi, config):
     systems = small_batch()
-    batch = capi.Batch(systems)
+    print("batch config", config, flush=True)
+    batch = capi.Batch(systems, config=config)
     info = batch.info()
     assert info["interactions"] == sum(s.interactions for s in systems)
     batch.zero()
