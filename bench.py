@@ -79,8 +79,20 @@ def cpu_baseline_leg(work, gpu_grid: np.ndarray, precision: dict | None, min_sec
         "sample": f"{done} of {work.grid} z-slices ({interactions:.3e} interactions, {elapsed:.1f} s) of the same "
                   f"256^3 x {work.atoms.shape[0]} workload; FP32 restatement of generate-coulomb-output.c, OpenMP",
     }
+    # C1 in full on the same restatement (2D slice, 256^2 x 4,000 atoms).
+    slice_work = workloads.slice2d()
+    start = time.perf_counter()
+    oracle.coulomb2d_f32(slice_work.slice_atoms, slice_work.spacing, slice_work.grid_x, slice_work.grid_y)
+    c1_seconds = time.perf_counter() - start
+    baseline["c1_full"] = {"value": slice_work.interactions / c1_seconds, "unit": "interactions/s",
+                           "sample": f"the whole C1 slice ({slice_work.interactions:.3e} interactions, {c1_seconds:.2f} s)"}
+    baseline["opencl_cpu_device"] = opencl_cpu_device()
     points = workloads.sample_points(33, 256, work.grid)
+    start = time.perf_counter()
     values, norms = oracle.points_f64(work.atoms, work.spacing, points)
+    fp64_seconds = time.perf_counter() - start
+    baseline["fp64_oracle"] = {"value": points.shape[0] * work.atoms.shape[0] / fp64_seconds, "unit": "interactions/s",
+                               "sample": f"{points.shape[0]} grid points of the headline workload (for information)"}
     worst, rms = oracle.normalised_error(gpu_grid[points[:, 2], points[:, 1], points[:, 0]], values, norms)
     errors = {"headline": {"max": worst, "rms": rms, "points": int(points.shape[0]), "against": "FP64 oracle"}}
     if precision is not None:
@@ -90,6 +102,17 @@ def cpu_baseline_leg(work, gpu_grid: np.ndarray, precision: dict | None, min_sec
             errors[f"precision_{form}"] = {"max": worst, "rms": rms, "points": int(precision["points"].shape[0]),
                                            "against": "FP64 oracle"}
     return baseline, errors
+
+
+def opencl_cpu_device() -> str:
+    """Whether an OpenCL CPU device could run the reference's .cl kernels beside the GPU: the
+    installed ICDs (/etc/OpenCL/vendors) are read, and no OpenCL runtime is loaded."""
+    vendors = sorted(Path("/etc/OpenCL/vendors").glob("*.icd"))
+    libraries = [path.read_text().strip() for path in vendors]
+    cpu = [name for name in libraries if any(key in name.lower() for key in ("pocl", "intelocl", "cpu"))]
+    if not cpu:
+        return "skipped: no OpenCL CPU device (installed ICDs: " + (", ".join(libraries) or "none") + ")"
+    return "present (" + ", ".join(cpu) + "), not run: this bench has no OpenCL host driver"
 
 
 def progress(message: str) -> None:
