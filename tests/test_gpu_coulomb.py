@@ -223,3 +223,61 @@ def test_clustered_cancellation_workload(capi, pairs, grid, centres):
     assert errors[capi.COMPENSATED][0] <= COMPENSATED_BAR
     assert errors[capi.PLAIN][0] <= PLAIN_BAR
     assert errors[capi.COMPENSATED][1] <= errors[capi.PLAIN][1]
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_single_point_grid_and_single_atom(capi, form):
+    """G = 1 (one grid point, at the origin) and N = 1 (V = q / r at every point), both forms."""
+    bar = PLAIN_BAR if form == 0 else COMPENSATED_BAR
+    atoms = workloads.uniform_atoms(27, 333, box=12.0)
+    one = run_volume(capi, atoms, 0.5, 1, form=form)
+    worst, _ = normalised(one, atoms, 0.5, np.zeros((1, 3), dtype=np.int32))
+    assert worst <= bar
+    charge = np.array([[3.3, 4.1, 5.7, 0.8]], dtype=np.float32)
+    grid = 24
+    result = run_volume(capi, charge, 0.5, grid, form=form)
+    points = all_points(grid)
+    d = points.astype(np.float64) * 0.5 - charge[0, :3].astype(np.float64)
+    exact = float(charge[0, 3]) / np.sqrt((d * d).sum(axis=1))
+    # One term: the FP32 displacement, r^2 and v_rsq_f32 are each within a few ulp.
+    np.testing.assert_allclose(result[points[:, 2], points[:, 1], points[:, 0]], exact, rtol=2e-6)
+
+
+@pytest.mark.parametrize("count", [1, 3, 127, 129, 255, 257])
+def test_atom_counts_around_the_tile(capi, count):
+    """Atom counts below, at and just past the default atom tiles (128 plain, 256 compensated):
+    the last tile is padded with zero charges. The reference's vector kernels drop such a tail
+    (Examples/CoulombSum3d/CoulombSum3d.cu:21); these kernels must not."""
+    atoms = workloads.uniform_atoms(280 + count, count, box=10.0)
+    points = all_points(20)
+    for form, bar in ((0, PLAIN_BAR), (1, COMPENSATED_BAR)):
+        print(f"count {count} form {form}", flush=True)
+        result = run_volume(capi, atoms, 0.5, 20, form=form)
+        worst, _ = normalised(result, atoms, 0.5, points)
+        assert worst <= bar, (count, form, worst)
+
+
+def test_translation_and_axis_swap(capi):
+    """Moving every atom by whole grid steps along x moves the map with it, and swapping the
+    atoms' x and y swaps the map's x and y axes, to the error bar of each form (per point,
+    normalised by sum_i |q_i| / r_i)."""
+    grid, spacing, shift = 32, 0.5, 6
+    atoms = workloads.uniform_atoms(29, 400, box=12.0)
+    # Coordinates on multiples of 2^-8 A: the shift and every displacement are exact in FP32, so
+    # the comparison sees the kernels' rounding only, not a rounding of the moved inputs.
+    atoms[:, :3] = np.round(atoms[:, :3] * 256.0) / 256.0
+    on_grid = np.all(np.mod(atoms[:, :3], spacing) == 0.0, axis=1)
+    atoms[on_grid, 2] += np.float32(1.0 / 256.0)
+    _, norms = oracle.points_f64(atoms, spacing, all_points(grid))
+    norms = norms.reshape(grid, grid, grid)
+    moved = atoms.copy()
+    moved[:, 0] += np.float32(shift * spacing)
+    swapped = np.ascontiguousarray(atoms[:, [1, 0, 2, 3]])
+    for form, bar in ((0, PLAIN_BAR), (1, COMPENSATED_BAR)):
+        base = run_volume(capi, atoms, spacing, grid, form=form)
+        shifted = run_volume(capi, moved, spacing, grid, form=form)
+        difference = np.abs(shifted[:, :, shift:].astype(np.float64) - base[:, :, :-shift]) / norms[:, :, :-shift]
+        assert difference.max() <= 2 * bar, (form, difference.max())
+        transposed = run_volume(capi, swapped, spacing, grid, form=form)
+        difference = np.abs(transposed.astype(np.float64) - base.transpose(0, 2, 1)) / norms.transpose(0, 2, 1)
+        assert difference.max() <= 2 * bar, (form, difference.max())
