@@ -835,45 +835,99 @@ int ktt_coulomb3d_iterative(const float* hostAtoms, int atomCount, float spacing
     auto kernel = runtime.GetKernel("CoulombSum3dIterative.hip", "directCoulombSum", parsed.Defines(kLaunchOnly));
     hipStream_t hipStream = static_cast<hipStream_t>(stream);
 
+    // Per slice, as the reference launcher: dz^2 of every atom in FP32 on the host
+    // (CoulombSum3dIterative.cpp:160-168), an upload of the whole atom array and a launch
+    // (:169-179). Two staging and two device buffers let the host prepare slice z + 1 while the
+    // GPU runs slice z, and the upload runs on a second stream beside that kernel; per-slice
+    // event pairs give the kernel time the reference reports.
     const size_t bytes = static_cast<size_t>(std::max(atomCount, 1)) * 4 * sizeof(float);
-    float* deviceAtoms = static_cast<float*>(runtime.GetScratch("iterative.atoms", bytes));
+    float* deviceBase = static_cast<float*>(runtime.GetScratch("iterative.atoms", 2 * bytes));
+    float* deviceAtoms[2] = {deviceBase, deviceBase + bytes / sizeof(float)};
     void* pinned = nullptr;
-    CheckHipError(hipHostMalloc(&pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
-    float* staging = static_cast<float*>(pinned);
-    std::memcpy(staging, hostAtoms, static_cast<size_t>(atomCount) * 4 * sizeof(float));
+    CheckHipError(hipHostMalloc(&pinned, 2 * bytes, hipHostMallocDefault), "hipHostMalloc");
+    float* staging[2] = {static_cast<float*>(pinned), static_cast<float*>(pinned) + bytes / sizeof(float)};
+    for (float* buffer : staging)
+    {
+        std::memcpy(buffer, hostAtoms, static_cast<size_t>(atomCount) * 4 * sizeof(float));
+    }
+    hipEvent_t events[6] = {};  // per slot: kernel start, kernel end, upload done
+    hipStream_t copyStream = nullptr;
+    const auto release = [&]()
+    {
+        IgnoreHipError(hipStreamSynchronize(hipStream));
+        if (copyStream != nullptr)
+        {
+            IgnoreHipError(hipStreamSynchronize(copyStream));
+            IgnoreHipError(hipStreamDestroy(copyStream));
+        }
+        for (hipEvent_t& event : events)
+        {
+            if (event != nullptr)
+            {
+                IgnoreHipError(hipEventDestroy(event));
+            }
+        }
+        IgnoreHipError(hipHostFree(pinned));
+    };
 
     const auto begin = std::chrono::steady_clock::now();
     float kernelTotal = 0.0f;
     try
     {
+        for (hipEvent_t& event : events)
+        {
+            CheckHipError(hipEventCreateWithFlags(&event, hipEventDefault), "hipEventCreateWithFlags");
+        }
+        CheckHipError(hipStreamCreateWithFlags(&copyStream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        bool pending[2] = {false, false};
+        // Waits for the kernel of slot s (and so its upload) and adds its time.
+        const auto retire = [&](const int s)
+        {
+            if (!pending[s])
+            {
+                return;
+            }
+            CheckHipError(hipEventSynchronize(events[3 * s + 1]), "hipEventSynchronize");
+            float sliceMs = 0.0f;
+            CheckHipError(hipEventElapsedTime(&sliceMs, events[3 * s], events[3 * s + 1]), "hipEventElapsedTime");
+            kernelTotal += sliceMs;
+            pending[s] = false;
+        };
         const float* unused = nullptr;
-        const float* atomsArgument = deviceAtoms;
         for (int z = 0; z < gridSize; ++z)
         {
-            // Host precompute of dz^2 in FP32 (CoulombSum3dIterative.cpp:160-168).
+            const int s = z & 1;
+            retire(s);
             const float coordinate = spacing * static_cast<float>(z);
+            float* slot = staging[s];
             for (int i = 0; i < atomCount; ++i)
             {
                 const float d = coordinate - hostAtoms[4 * i + 2];
-                staging[4 * i + 2] = d * d;
+                slot[4 * i + 2] = d * d;
             }
-            CheckHipError(hipMemcpyAsync(deviceAtoms, staging, static_cast<size_t>(atomCount) * 4 * sizeof(float), hipMemcpyHostToDevice,
-                hipStream), "hipMemcpyAsync");
-            CheckHipError(hipStreamSynchronize(hipStream), "hipStreamSynchronize");
+            // The kernel of slot s has ended (retire), so its device buffer is free for the upload.
+            CheckHipError(hipMemcpyAsync(deviceAtoms[s], slot, static_cast<size_t>(atomCount) * 4 * sizeof(float), hipMemcpyHostToDevice,
+                copyStream), "hipMemcpyAsync");
+            CheckHipError(hipEventRecord(events[3 * s + 2], copyStream), "hipEventRecord");
+            CheckHipError(hipStreamWaitEvent(hipStream, events[3 * s + 2], 0), "hipStreamWaitEvent");
+            const float* atomsArgument = deviceAtoms[s];
             int zIndex = z;
             void* arguments[] = {&atomsArgument, &unused, &unused, &unused, &unused, &atomCount, &spacing, &zIndex, &energyGrid};
-            float sliceMs = 0.0f;
-            CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, &sliceMs);
-            kernelTotal += sliceMs;
+            CheckHipError(hipEventRecord(events[3 * s], hipStream), "hipEventRecord");
+            CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, nullptr);
+            CheckHipError(hipEventRecord(events[3 * s + 1], hipStream), "hipEventRecord");
+            pending[s] = true;
         }
+        retire(gridSize & 1);
+        retire((gridSize + 1) & 1);
     }
     catch (...)
     {
-        IgnoreHipError(hipHostFree(pinned));
+        release();
         throw;
     }
     const auto finish = std::chrono::steady_clock::now();
-    IgnoreHipError(hipHostFree(pinned));
+    release();
     if (kernelMs != nullptr)
     {
         *kernelMs = kernelTotal;
