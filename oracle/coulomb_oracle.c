@@ -34,7 +34,8 @@
 void oracle_coulomb3d_f32(const float* atoms, int n, float gs, int cells, int z0, int z1, float* grid)
 {
     const long long plane = (long long)cells * cells;
-#pragma omp parallel for schedule(dynamic, 1)
+    /* Rows (z, y) are independent: parallel over both, so that a single slice uses every thread. */
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
     for (int z = z0; z < z1; z++)
     {
         for (int y = 0; y < cells; y++)
@@ -42,14 +43,18 @@ void oracle_coulomb3d_f32(const float* atoms, int n, float gs, int cells, int z0
             for (int x = 0; x < cells; x++)
             {
                 float* out = &grid[(long long)z * plane + (long long)y * cells + x];
+                /* The same additions in the same order as `*out += e` per atom, kept in a register
+                 * (the grid cannot alias the atoms, but the compiler cannot know it). */
+                float acc = *out;
                 for (int at = 0; at < n; at++)
                 {
                     const float dx = (float)x * gs - atoms[at * 4 + 0];
                     const float dy = (float)y * gs - atoms[at * 4 + 1];
                     const float dz = (float)z * gs - atoms[at * 4 + 2];
                     const float e = atoms[at * 4 + 3] / sqrtf(dx * dx + dy * dy + dz * dz);
-                    *out += e;
+                    acc += e;
                 }
+                *out = acc;
             }
         }
     }
