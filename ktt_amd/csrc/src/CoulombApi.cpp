@@ -358,7 +358,10 @@ static CoulombConfig Defaults3d(const int form, const long gridSize)
         {
             wgx = 64;
         }
-        const double partialBytes = 4.0 * 8.0 * static_cast<double>(gridSize) * gridSize * gridSize;
+        // Partials cover the launch: x padded to whole spans of 2 * wgx points, rows to 16.
+        const double paddedX = static_cast<double>((gridSize + 2 * wgx - 1) / (2 * wgx) * (2 * wgx));
+        const double paddedRows = static_cast<double>((static_cast<long>(gridSize) * gridSize + 15) / 16 * 16);
+        const double partialBytes = 4.0 * 8.0 * paddedX * paddedRows;
         const char* blocks = partialBytes <= 2.0 * 1024 * 1024 * 1024 ? "4" : "1";
         config.values = {{"WORK_GROUP_SIZE_X", std::to_string(wgx)}, {"WORK_GROUP_SIZE_Y", "2"}, {"POINTS_X", "2"},
             {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "256"}, {"INNER_UNROLL_FACTOR", "8"}, {"ATOM_BLOCKS", blocks}};
