@@ -287,6 +287,18 @@ CoulombShape CoulombRuntime::RowShape(const CoulombConfig& config, const long sp
     return shape;
 }
 
+size_t CoulombRuntime::ResidentWorkgroups(const HipKernel& kernel, const int threads)
+{
+    int device = 0;
+    CheckHipError(hipGetDevice(&device), "hipGetDevice");
+    int computeUnits = 0;
+    CheckHipError(hipDeviceGetAttribute(&computeUnits, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+    int perUnit = 0;
+    CheckHipError(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perUnit, kernel.GetFunction(), threads, 0),
+        "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    return static_cast<size_t>(std::max(computeUnits, 0)) * static_cast<size_t>(std::max(perUnit, 0));
+}
+
 void CoulombRuntime::Launch(const HipKernel& kernel, const unsigned int grid[3], const unsigned int block[3], void** arguments,
     hipStream_t stream, float* kernelMs)
 {
@@ -343,13 +355,9 @@ static CoulombConfig Defaults3d(const int form, const long gridSize)
     // prefers one point per thread and 8 rows.  A row length that is not a multiple of the span
     // narrows the workgroup to 64 threads in x, so that fewer lanes idle past the row's end.
     //
-    // With one atom block, 5 waves per SIMD leave the last of 3.2 rounds of workgroups a fifth
-    // full; splitting the atoms over 4 blocks (12.8 rounds) is 2.8% faster.  The compensated
-    // blocks meet in (sum, compensation) partials merged in FP64 in a fixed order (deterministic):
-    // 8 bytes per point and block, written and read back, 1.07 GB of HBM traffic at 256^3 against
-    // 0.14 GB without the split, at 0.02% of the HBM bandwidth.  The split is kept while the
-    // partials stay within 2 GiB.  The plain form keeps one block: its blocks would add with FP32 atomics, whose order
-    // is not fixed.
+    // TAIL_SPLIT (compensated form, one atom block): the tiles of the last, partial round of
+    // workgroups split their atoms so that the round fills the GPU (see ktt_coulomb3d).  The
+    // plain form has no split: its blocks would add with FP32 atomics, whose order is not fixed.
     CoulombConfig config;
     long wgx = 128;
     if (form == KTT_COULOMB_COMPENSATED)
@@ -358,13 +366,8 @@ static CoulombConfig Defaults3d(const int form, const long gridSize)
         {
             wgx = 64;
         }
-        // Partials cover the launch: x padded to whole spans of 2 * wgx points, rows to 16.
-        const double paddedX = static_cast<double>((gridSize + 2 * wgx - 1) / (2 * wgx) * (2 * wgx));
-        const double paddedRows = static_cast<double>((static_cast<long>(gridSize) * gridSize + 15) / 16 * 16);
-        const double partialBytes = 4.0 * 8.0 * paddedX * paddedRows;
-        const char* blocks = partialBytes <= 2.0 * 1024 * 1024 * 1024 ? "4" : "1";
         config.values = {{"WORK_GROUP_SIZE_X", std::to_string(wgx)}, {"WORK_GROUP_SIZE_Y", "2"}, {"POINTS_X", "2"},
-            {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "256"}, {"INNER_UNROLL_FACTOR", "8"}, {"ATOM_BLOCKS", blocks}};
+            {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "256"}, {"INNER_UNROLL_FACTOR", "8"}, {"ATOM_BLOCKS", "1"}, {"TAIL_SPLIT", "1"}};
     }
     else
     {
@@ -390,7 +393,10 @@ static CoulombConfig DefaultsSlice()
     return config;
 }
 
-static const std::vector<std::string> kLaunchOnly = {"ATOM_BLOCKS"};
+static const std::vector<std::string> kLaunchOnly = {"ATOM_BLOCKS", "TAIL_SPLIT"};
+
+// Most blocks a tail tile is split over (the tail round then holds at most this many times its tiles).
+constexpr long kMaxTailBlocks = 16;
 
 static void CheckAtoms(const float* atoms, const int atomCount)
 {
@@ -736,16 +742,33 @@ int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize
 
     auto kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSumCompensated", defines);
     const size_t tiles = static_cast<size_t>(shape.grid[0]) * shape.grid[1];
-    void* partials = nullptr;
-    if (atomBlocks > 1)
+    // ATOM_BLOCKS > 1 splits every tile.  Otherwise, with TAIL_SPLIT, only the tiles of the last
+    // round of workgroups are split: with S resident workgroups per launch (CUs x workgroups per
+    // CU at this kernel's occupancy), tiles % S tiles are left for that round, and splitting each
+    // over about S / (tiles % S) blocks fills it.
+    int splitFrom = 0;
+    long blocks = atomBlocks;
+    if (atomBlocks == 1 && parsed.Get("TAIL_SPLIT", 0) != 0)
     {
-        const size_t bytes = static_cast<size_t>(atomBlocks) * tiles * static_cast<size_t>(shape.threads)
+        const size_t slots = CoulombRuntime::ResidentWorkgroups(*kernel, shape.threads);
+        const size_t tail = slots > 0 ? tiles % slots : 0;
+        const long atomTiles = std::max<long>(1, (atomCount + parsed.Get("ATOM_TILE", 64) - 1) / parsed.Get("ATOM_TILE", 64));
+        const long fill = tail > 0 ? static_cast<long>(slots / tail) : 1;
+        blocks = std::max<long>(1, std::min<long>({fill, atomTiles, kMaxTailBlocks}));
+        splitFrom = blocks > 1 ? static_cast<int>(tiles - tail) : static_cast<int>(tiles);
+    }
+    unsigned int grid[3] = {shape.grid[0], shape.grid[1], static_cast<unsigned int>(blocks)};
+    void* partials = nullptr;
+    if (blocks > 1)
+    {
+        const size_t bytes = static_cast<size_t>(blocks) * (tiles - static_cast<size_t>(splitFrom)) * static_cast<size_t>(shape.threads)
             * static_cast<size_t>(shape.pointsPerThread) * 8;
         partials = runtime.GetScratch("coulomb3d.partials", bytes);
     }
     void* counters = runtime.GetScratch("coulomb3d.counters", tiles * sizeof(unsigned int));
-    void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid, &partials, &counters};
-    CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, kernelMs);
+    void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid, &partials, &counters,
+        &splitFrom};
+    CoulombRuntime::Launch(*kernel, grid, shape.block, arguments, hipStream, kernelMs);
     return 0;
     KTT_C_CATCH
 }

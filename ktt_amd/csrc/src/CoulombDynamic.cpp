@@ -197,8 +197,10 @@ extern "C" int ktt_coulomb3d_dynamic(const float* hostAtoms, int atomCount, floa
         const ArgumentId sigmaId = tuner.AddArgumentScalar(sigma);
         tuner.SetArguments(jitter, {atomsId, countId, seedId, stepId, sigmaId});
         tuner.SetArguments(zero, {gridId, wordsId});
+        // splitFrom 0: ATOM_BLOCKS splits every tile, as the tuning space means it.
+        const ArgumentId splitFromId = tuner.AddArgumentScalar(0);
         tuner.SetArguments(coulomb, {atomsId, soaIds[0], soaIds[1], soaIds[2], soaIds[3], countId, spacingId, sideId, gridId,
-            partialsId, countersId});
+            partialsId, countersId, splitFromId});
 
         const KernelId kernel = tuner.CreateCompositeKernel("CoulombSum3dDynamic", {jitter, zero, coulomb});
         AddCoulombDynamicTuningSpace(tuner, kernel, coulomb, side);

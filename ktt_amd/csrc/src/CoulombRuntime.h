@@ -53,6 +53,9 @@ public:
     void* GetScratch(const std::string& name, const size_t bytes);
 
     static CoulombShape RowShape(const CoulombConfig& config, const long spanX, const long rows, const long atomBlocks);
+    /** Workgroups of `threads` threads that the current device holds at once for this kernel:
+     *  compute units x workgroups per compute unit at the kernel's occupancy. */
+    static size_t ResidentWorkgroups(const HipKernel& kernel, const int threads);
     static void Launch(const HipKernel& kernel, const unsigned int grid[3], const unsigned int block[3], void** arguments,
         hipStream_t stream, float* kernelMs);
 

@@ -60,16 +60,22 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const 
 // workgroup's first x, formed in FP64), a two-sum per point per atom tile, and
 // an FP64 merge of the atom-block partials inside this single launch.
 //
-//   partials     float2[atomBlocks * tiles * threads * points], (sum, comp)
-//                per point; tiles = gridDim.x * gridDim.y.  Unused when
-//                gridDim.z == 1.
+// Tiles are numbered blockIdx.y * gridDim.x + blockIdx.x.  Tiles below
+// splitFrom take the whole atom range in block 0 and store directly; their
+// other blocks exit at once.  Tiles from splitFrom on split the atom range over
+// the gridDim.z blocks and meet in partials:
+//   partials     float2[gridDim.z * (tiles - splitFrom) * threads * points],
+//                (sum, comp) per point.  Unused when gridDim.z == 1.
 //   tileCounters uint[tiles], zero before the first launch; every launch
 //                leaves them zero again.
-// The last workgroup of a tile to finish (counted with a device-scope atomic)
-// sums the partials of all atom blocks in FP64 and adds them to energyGrid.
+// The last workgroup of a split tile to finish (counted with a device-scope
+// atomic) sums the partials of all its blocks in FP64 and adds them to
+// energyGrid.  splitFrom = 0 splits every tile; splitting only the tiles of the
+// last, partial round of workgroups keeps every slot of the GPU busy to the end
+// while the other tiles need no partials.
 extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompensated(const float4* atomInfo,
     const float* atomInfoX, const float* atomInfoY, const float* atomInfoZ, const float* atomInfoW, int numberOfAtoms,
-    float gridSpacing, int gridSize, float* energyGrid, float2* partials, unsigned int* tileCounters)
+    float gridSpacing, int gridSize, float* energyGrid, float2* partials, unsigned int* tileCounters, int splitFrom)
 {
     __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
@@ -80,9 +86,19 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
     const int xFirst = threadIdx.x * POINTS_X;
     const int x0 = xBase + xFirst;
     const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
+    const size_t tile = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+    const bool split = gridDim.z > 1 && tile >= static_cast<size_t>(splitFrom);
+    if (gridDim.z > 1 && !split && blockIdx.z != 0)
+    {
+        return;  // a whole tile: block 0 takes every atom
+    }
 
-    int a0, a1;
-    kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, ATOM_TILE, a0, a1);
+    int a0 = 0;
+    int a1 = numberOfAtoms;
+    if (split)
+    {
+        kc::atomBlockRange(numberOfAtoms, blockIdx.z, gridDim.z, ATOM_TILE, a0, a1);
+    }
 
     const double h = static_cast<double>(gridSpacing);
     const double originX = h * xBase;
@@ -103,18 +119,18 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
     E::accumulate<kc::Volume, true>(shared, src, a0, a1, gridSpacing, tid, rowBase, rowsTotal, gridSize, originX, xLocal, part,
         sum, comp);
 
-    if (gridDim.z == 1)
+    if (!split)
     {
         E::storeCompensated(energyGrid, gridSize, 0, gridSize, rowsTotal, x0, row0, sum, comp);
         return;
     }
 
-    // Partials laid out [block][tile][point-in-thread][thread] so that the
-    // lanes of a wave store consecutive float2s.
-    const size_t tiles = static_cast<size_t>(gridDim.x) * gridDim.y;
-    const size_t tile = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+    // Partials of the split tiles laid out [block][split tile][point-in-thread][thread]
+    // so that the lanes of a wave store consecutive float2s.
+    const size_t splitTiles = static_cast<size_t>(gridDim.x) * gridDim.y - static_cast<size_t>(splitFrom);
+    const size_t slot = tile - static_cast<size_t>(splitFrom);
     const size_t tileStride = static_cast<size_t>(E::Threads) * E::Points;
-    float2* mine = partials + (static_cast<size_t>(blockIdx.z) * tiles + tile) * tileStride;
+    float2* mine = partials + (static_cast<size_t>(blockIdx.z) * splitTiles + slot) * tileStride;
 #pragma unroll
     for (int k = 0; k < ROWS_PER_THREAD; ++k)
     {
@@ -151,7 +167,7 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSumCompens
             double total = 0.0;
             for (unsigned int b = 0; b < gridDim.z; ++b)
             {
-                const float2 sc = partials[(static_cast<size_t>(b) * tiles + tile) * tileStride
+                const float2 sc = partials[(static_cast<size_t>(b) * splitTiles + slot) * tileStride
                     + (k * POINTS_X + i) * E::Threads + tid];
                 total += static_cast<double>(sc.x) + static_cast<double>(sc.y);
             }

@@ -281,3 +281,26 @@ def test_translation_and_axis_swap(capi):
         transposed = run_volume(capi, swapped, spacing, grid, form=form)
         difference = np.abs(transposed.astype(np.float64) - base.transpose(0, 2, 1)) / norms.transpose(0, 2, 1)
         assert difference.max() <= 2 * bar, (form, difference.max())
+
+
+def test_compensated_tail_split(capi):
+    """160^3 with the compensated default (64-thread spans of 128 points, 16 rows): 3,200 tiles,
+    more than the workgroups 256 CUs hold at once, so only the tiles of the last, partial round
+    split their atoms and meet in FP64 partials; the others store directly. The last z-planes
+    (split tiles) and sampled points everywhere meet the bar, a second launch is bitwise equal
+    (the merge order is fixed), and TAIL_SPLIT=0 (no split) agrees to twice the bar."""
+    atoms = workloads.uniform_atoms(30, 600, box=80.0)
+    grid, spacing = 160, 0.5
+    first = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED)
+    second = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED)
+    np.testing.assert_array_equal(first, second)
+    z, y, x = np.meshgrid(np.arange(grid - 2, grid), np.arange(grid), np.arange(grid), indexing="ij")
+    last_planes = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+    for points in (last_planes, workloads.sample_points(31, 4096, grid)):
+        worst, _ = normalised(first, atoms, spacing, points)
+        assert worst <= COMPENSATED_BAR, worst
+    whole = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED, config="TAIL_SPLIT=0")
+    _, norms = oracle.points_f64(atoms, spacing, last_planes)
+    got = first[last_planes[:, 2], last_planes[:, 1], last_planes[:, 0]].astype(np.float64)
+    reference = whole[last_planes[:, 2], last_planes[:, 1], last_planes[:, 0]]
+    assert (np.abs(got - reference) / norms).max() <= 2 * COMPENSATED_BAR
