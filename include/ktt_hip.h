@@ -85,7 +85,9 @@ int ktt_hip_event_elapsed_ms(void* start, void* end, float* milliseconds);
  * Replaces the launch of directCoulombSum, Examples/CoulombSum3d/CoulombSum3d.cu:1-47
  * (driven by Examples/CoulombSum3d/CoulombSum3d.cpp:117-237).
  * atoms: float4[n_atoms] on the device. Config names: WORK_GROUP_SIZE_X, WORK_GROUP_SIZE_Y,
- * POINTS_X, ROWS_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR, ATOM_BLOCKS. */
+ * POINTS_X, ROWS_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR, ATOM_BLOCKS (atom blocks for every
+ * tile), TAIL_SPLIT (compensated form with one atom block: 1, the default, splits the tiles of
+ * the last, partial round of workgroups over atom blocks; 0 does not). */
 int ktt_coulomb3d(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                   int form, const char* config, void* stream, float* kernel_ms);
 
