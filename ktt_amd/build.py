@@ -57,9 +57,9 @@ KERNEL_CHECKS = [
     ("CoulombSum3d.hip", "directCoulombSumCompensated",
      "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=8,POINTS_X=6,ROWS_PER_THREAD=4,ATOM_TILE=128"),
     ("CoulombSum3d.hip", "directCoulombSum",
-     "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=8,ATOM_TILE=128,INNER_UNROLL_FACTOR=8"),
+     "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=8,ATOM_TILE=128,INNER_UNROLL_FACTOR=16"),
     ("CoulombSum3d.hip", "directCoulombSumCompensated",
-     "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=256,INNER_UNROLL_FACTOR=8"),
+     "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=256,INNER_UNROLL_FACTOR=16"),
     ("Nbody.hip", "nbody_kernel", None),
     ("Nbody.hip", "nbody_kernel", "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=8,ATOM_TILE=512,INNER_UNROLL_FACTOR=8,USE_SOA=1"),
     ("NbodyReference.hip", "nbody_kernel_reference", None),
