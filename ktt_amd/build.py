@@ -46,9 +46,9 @@ KERNEL_CHECKS = [
     ("CoulombSum2dReference.hip", "directCoulombSumReference", None),
     ("CoulombSum3dIterative.hip", "directCoulombSum", "POINTS_X=2,ROWS_PER_THREAD=8"),
     ("CoulombSum3dIterative.hip", "directCoulombSumSlice", "POINTS_X=2,ROWS_PER_THREAD=8"),
-    ("CoulombSum2d.hip", "directCoulombSum", "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=8"),
-    ("CoulombSum3dIterative.hip", "directCoulombSum", "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=8"),
-    ("CoulombSum3dIterative.hip", "directCoulombSumSlice", "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=8"),
+    ("CoulombSum2d.hip", "directCoulombSum", "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=16"),
+    ("CoulombSum3dIterative.hip", "directCoulombSum", "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=16"),
+    ("CoulombSum3dIterative.hip", "directCoulombSumSlice", "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=16"),
     ("CoulombSum3dIterativeReference.hip", "directCoulombSumReference", None),
     ("coulomb_util.hip", "kttZeroFill16", None),
     ("coulomb_util.hip", "kttCopy16", None),

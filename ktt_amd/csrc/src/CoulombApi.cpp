@@ -384,12 +384,12 @@ static CoulombConfig Defaults3d(const int form, const long gridSize)
 static CoulombConfig DefaultsSlice()
 {
     // One slice (256 x 256 points) has too few points to fill 256 CUs with several waves per
-    // SIMD, so the atoms are split over up to 128 blocks (fewer when there are fewer atom tiles)
+    // SIMD, so the atoms are split over up to 256 blocks (fewer when there are fewer atom tiles)
     // and every thread keeps a small 1 x 4 block: at 256^3 x 65,536 the 256 slice launches run
-    // at 70% of the FP32 peak (44% with 16 blocks of 2 x 8 points and no unroll).
+    // at 72% of the FP32 peak (44% with 16 blocks of 2 x 8 points and no unroll).
     CoulombConfig config;
     config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "1"}, {"ROWS_PER_THREAD", "4"},
-        {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "8"}, {"ATOM_BLOCKS", "128"}};
+        {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ATOM_BLOCKS", "256"}};
     return config;
 }
 
