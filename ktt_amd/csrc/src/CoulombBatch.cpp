@@ -630,7 +630,7 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
     tuner.AddParameter(kernel, prefix + "PX", std::vector<uint64_t>{1, 2, 3, 4});
     tuner.AddParameter(kernel, prefix + "ROWS", std::vector<uint64_t>{2, 4, 8});
     tuner.AddParameter(kernel, prefix + "TILE", std::vector<uint64_t>{32, 64, 128});
-    tuner.AddParameter(kernel, prefix + "UNROLL", std::vector<uint64_t>{1, 8});
+    tuner.AddParameter(kernel, prefix + "UNROLL", std::vector<uint64_t>{1, 8, 16});
     // Every other bucket keeps its current shape (single-valued parameters are defines too).
     for (int b = 0; b < kMaxBuckets; ++b)
     {
@@ -721,7 +721,7 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
         return std::find(list.begin(), list.end(), value) != list.end();
     };
     if (inList(incumbent.wgx, {16, 32, 64}) && inList(incumbent.px, {1, 2, 3, 4}) && inList(incumbent.rows, {2, 4, 8})
-        && inList(incumbent.tile, {32, 64, 128}) && inList(incumbent.unroll, {1, 8}))
+        && inList(incumbent.tile, {32, 64, 128}) && inList(incumbent.unroll, {1, 8, 16}))
     {
         incumbentResult = tuner.Run(kernel, tuner.CreateConfiguration(kernel, incumbentInput), {});
         if (incumbentResult.IsValid() && (winner == nullptr || incumbentResult.GetKernelDuration() < winner->GetKernelDuration()))
