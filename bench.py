@@ -426,7 +426,7 @@ def main() -> int:
             "interactions_per_step": work.interactions,
             "form": "compensated",
             "kernel_config": args.config or "default",
-            "device": (info["name"] + " " + info["arch"]).strip(),
+            "device": info["name"] if info["arch"] in info["name"] else (info["name"] + " " + info["arch"]).strip(),
         },
         "plain": results["plain"],
         "compensated": results["compensated"],
