@@ -115,6 +115,14 @@ int ktt_coulomb2d_reference(const float* atoms, int n_atoms, float spacing, int 
 int ktt_coulomb3d_iterative(const float* host_atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                             const char* config, void* stream, float* kernel_ms, float* wall_ms);
 
+/* The iterative example's validation kernel, one point per thread over the whole G^3 grid
+ * with the atoms' real z, "+=" store
+ * (Examples/CoulombSum3dIterative/CoulombSum3dIterativeReference.cl:1-26, launched over
+ * (G, G, G) in workgroups of 16 x 16 x 1 as CoulombSum3dIterative.cpp:79-81 does); grid_size
+ * a positive multiple of 16. */
+int ktt_coulomb3d_iterative_reference(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
+                                      void* stream, float* kernel_ms);
+
 /* Fused slice loop: the slice kernel forms dz^2 on the device (atoms with their real z on
  * the device), one launch per slice, no host work between launches. */
 int ktt_coulomb3d_slices(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,

@@ -52,6 +52,7 @@ SIGNATURES = {
     "ktt_coulomb2d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _vp, _fp]),
     "ktt_coulomb3d_iterative": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_char_p, _vp, _fp, _fp]),
     "ktt_coulomb3d_slices": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_char_p, _vp, _fp]),
+    "ktt_coulomb3d_iterative_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _vp, _fp]),
     "ktt_coulomb_launch_shape": (_c.c_int, [_c.c_int, _c.c_int, _c.c_char_p, _uint3, _uint3]),
     "ktt_coulomb3d_tune": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_char_p,
                                       _c.c_char_p, _c.c_size_t, _c.POINTER(_c.c_double), _c.POINTER(_c.c_int),
@@ -273,6 +274,14 @@ def coulomb3d_slices(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: in
     ms = ctypes.c_float(0.0)
     _check(library().ktt_coulomb3d_slices(atoms.ptr, n_atoms, spacing, grid, energy.ptr, _config(config), stream,
                                           ctypes.byref(ms) if timed else None), "ktt_coulomb3d_slices")
+    return ms.value if timed else None
+
+
+def coulomb3d_iterative_reference(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, energy: DeviceBuffer,
+                                  stream=None, timed: bool = True) -> float | None:
+    ms = ctypes.c_float(0.0)
+    _check(library().ktt_coulomb3d_iterative_reference(atoms.ptr, n_atoms, spacing, grid, energy.ptr, stream,
+                                                       ctypes.byref(ms) if timed else None), "ktt_coulomb3d_iterative_reference")
     return ms.value if timed else None
 
 

@@ -841,6 +841,26 @@ int ktt_coulomb2d_reference(const float* atoms, int atomCount, float spacing, in
     KTT_C_CATCH
 }
 
+int ktt_coulomb3d_iterative_reference(const float* atoms, int atomCount, float spacing, int gridSize, float* energyGrid,
+    void* stream, float* kernelMs)
+{
+    KTT_C_TRY
+    CheckAtoms(atoms, atomCount);
+    if (gridSize <= 0 || gridSize % 16 != 0 || static_cast<long>(gridSize) * gridSize > 0x7fffffffL)
+    {
+        return SetLastError("the iterative reference kernel needs a grid size that is a positive multiple of 16 (at most 46336)");
+    }
+    CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
+    auto kernel = CoulombRuntime::Get().GetKernel("CoulombSum3dIterativeReference.hip", "directCoulombSumReference", {});
+    const unsigned int grid[3] = {static_cast<unsigned int>(gridSize / 16), static_cast<unsigned int>(gridSize / 16),
+        static_cast<unsigned int>(gridSize)};
+    const unsigned int block[3] = {16, 16, 1};
+    void* arguments[] = {&atoms, &atomCount, &spacing, &energyGrid};
+    CoulombRuntime::Launch(*kernel, grid, block, arguments, static_cast<hipStream_t>(stream), kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
 static void CheckSliceGrid(const CoulombShape& shape, const CoulombConfig& parsed, const int gridSize)
 {
     const long spanX = static_cast<long>(shape.block[0]) * parsed.Get("POINTS_X", 4);

@@ -100,6 +100,37 @@ def test_bucket_tuning_keeps_results_correct(capi, tmp_path):
     check_against_oracle(systems, batch.grids(), COMPENSATED_BAR)
 
 
+def test_full_batch_every_system_against_oracle(capi):
+    """The 1,024-system batch of the benchmark (atom counts log-uniform in [200, 20000], sides
+    32/48/64/96/128, so every bucket's tile shape runs) through the graph-launched persistent
+    kernel: every system at 64 sampled points against FP64, compensated bar; a graph replay
+    equals the first run bit for bit."""
+    systems = workloads.batch()
+    assert sorted({s.grid for s in systems}) == list(workloads.BATCH_SIDES)
+    batch = capi.Batch(systems)
+    info = batch.info()
+    print("full batch:", len(systems), "systems,", info["items"], "items,", info["shapes"], flush=True)
+    assert info["interactions"] == sum(s.interactions for s in systems)
+    batch.zero()
+    ms = batch.run(use_graph=True)
+    first = [g.copy() for g in batch.grids()]
+    worst_by_side = {}
+    rms_all = []
+    for k, (system, grid) in enumerate(zip(systems, first)):
+        points = workloads.sample_points(4000 + k, 64, system.grid)
+        values, norms = oracle.points_f64(system.atoms, system.spacing, points)
+        worst, rms = oracle.normalised_error(grid[points[:, 2], points[:, 1], points[:, 0]], values, norms)
+        worst_by_side[system.grid] = max(worst_by_side.get(system.grid, 0.0), worst)
+        rms_all.append(rms)
+    print(f"full batch {ms:.1f} ms; worst normalised error by side {worst_by_side}; mean rms {np.mean(rms_all):.2e}", flush=True)
+    assert max(worst_by_side.values()) <= COMPENSATED_BAR
+    batch.zero()
+    batch.run(use_graph=True)
+    for k, grid in enumerate(batch.grids()):
+        np.testing.assert_array_equal(grid, first[k], err_msg=f"replay, system {k}")
+    batch.close()
+
+
 def test_batch_rejects_bad_input(capi):
     with pytest.raises(capi.KttHipError):
         capi.Batch([workloads.BatchSystem(np.zeros((3, 4), np.float32), side, 0.5) for side in (8, 16, 24, 32, 40, 56)])

@@ -113,6 +113,69 @@ def test_reference_kernel_matches_oracle(capi):
     assert worst <= PLAIN_BAR
 
 
+def test_slice_2d_reference_kernel_matches_oracle(capi):
+    """CoulombSum2dReference (one point per thread, "=" store) on the 256^2 x 4,000 slice
+    against the oracle's FP32 restatement of Examples/CoulombSum2d/CoulombSum2dReference.cl:8-20
+    (SideBySide 0.01) and against FP64 at every point; the "=" store overwrites what the
+    output held."""
+    work = workloads.slice2d()
+    atoms = capi.DeviceBuffer.from_array(work.slice_atoms)
+    energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
+    energy.upload(np.full((work.grid_y, work.grid_x), 7.0, dtype=np.float32))
+    capi.coulomb2d_reference(atoms, work.slice_atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy)
+    result = energy.download((work.grid_y, work.grid_x))
+    cpu = oracle.coulomb2d_f32(work.slice_atoms, work.spacing, work.grid_x, work.grid_y)
+    assert np.max(np.abs(result - cpu)) <= SIDE_BY_SIDE
+    y, x = np.meshgrid(np.arange(work.grid_y), np.arange(work.grid_x), indexing="ij")
+    points = np.stack([x.ravel(), y.ravel()], axis=1).astype(np.int32)
+    values, norms = oracle.points2d_f64(work.slice_atoms, work.spacing, points)
+    worst, rms = oracle.normalised_error(result[points[:, 1], points[:, 0]], values, norms)
+    print(f"2d reference kernel: error max {worst:.2e} rms {rms:.2e}", flush=True)
+    assert worst <= PLAIN_BAR
+
+
+def test_iterative_reference_kernel_matches_oracle(capi):
+    """CoulombSum3dIterativeReference (one point per thread over the whole volume with the
+    atoms' real z, "+=" store) against the FP64 oracle at every point and the FP32 restatement
+    of the reference generator (SideBySide 0.01), added onto a non-zero grid."""
+    atoms = workloads.uniform_atoms(34, 300, box=16.0)
+    grid, spacing = 32, 0.5
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    start = np.full((grid, grid, grid), -1.25, dtype=np.float32)
+    energy.upload(start)
+    capi.coulomb3d_iterative_reference(atoms_buffer, atoms.shape[0], spacing, grid, energy)
+    result = energy.download((grid, grid, grid))
+    cpu = oracle.coulomb3d_f32(atoms, spacing, grid, out=start.copy())
+    assert np.max(np.abs(result - cpu)) <= SIDE_BY_SIDE
+    worst, rms = normalised(result - start, atoms, spacing, all_points(grid))
+    print(f"iterative reference kernel: error max {worst:.2e} rms {rms:.2e}", flush=True)
+    assert worst <= PLAIN_BAR
+    with pytest.raises(capi.KttHipError, match="multiple of 16"):
+        capi.coulomb3d_iterative_reference(atoms_buffer, atoms.shape[0], spacing, 40, energy)
+
+
+@pytest.mark.parametrize("config", [None, "ATOM_BLOCKS=1", "WORK_GROUP_SIZE_X=32,POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=32,ATOM_BLOCKS=5",
+                                    "WORK_GROUP_SIZE_X=16,WORK_GROUP_SIZE_Y=8,POINTS_X=4,ROWS_PER_THREAD=2,ATOM_TILE=64,"
+                                    "INNER_UNROLL_FACTOR=4,ATOM_BLOCKS=3"])
+def test_fused_slice_kernel_matches_oracle(capi, config):
+    """directCoulombSumSlice (dz^2 formed on the device, one launch per slice) against the FP64
+    oracle at every point of a 64^3 grid, with one and several atom blocks (FP32 atomics)."""
+    atoms = workloads.uniform_atoms(35, 1100, box=32.0)
+    grid, spacing = 64, 0.5
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    energy.zero()
+    print(f"fused slices config={config}", flush=True)
+    capi.coulomb3d_slices(atoms_buffer, atoms.shape[0], spacing, grid, energy, config=config)
+    result = energy.download((grid, grid, grid))
+    worst, rms = normalised(result, atoms, spacing, all_points(grid))
+    print(f"  error max {worst:.2e} rms {rms:.2e}", flush=True)
+    assert worst <= PLAIN_BAR
+    cpu = oracle.coulomb3d_f32(atoms, spacing, grid)
+    assert np.max(np.abs(result - cpu)) <= SIDE_BY_SIDE
+
+
 def test_accumulates_into_grid_and_empty_atoms_leave_it(capi):
     atoms = workloads.uniform_atoms(25, 100, box=8.0)
     grid, spacing = 16, 0.5
@@ -179,10 +242,40 @@ def test_iterative_and_fused_slices_match_volume(capi, config):
     assert np.max(np.abs(fused - volume)) <= SIDE_BY_SIDE
 
 
-def test_full_size_volume_properties(capi):
-    """256^3 x 65,536 atoms: FP64 error at sampled points, linearity and sign flip."""
+def test_full_size_iterative_forms(capi):
+    """CoulombSum3dIterative at 256^3 x 65,536 atoms, both launchers with their default slice
+    shape (256 atom blocks per slice adding with FP32 atomics): the reference's host dz^2
+    scheme (Examples/CoulombSum3dIterative/CoulombSum3dIterative.cpp:146-184) and the fused
+    device dz^2, each against FP64 at 4,096 sampled points and against the volume kernel
+    (SideBySide 0.01) everywhere."""
     work = workloads.volume3d()
-    points = workloads.sample_points(33, 256, work.grid)
+    points = workloads.sample_points(36, 4096, work.grid)
+    values, norms = oracle.points_f64(work.atoms, work.spacing, points)
+    atoms_buffer = capi.DeviceBuffer.from_array(work.atoms)
+    energy = capi.DeviceBuffer(4 * work.grid ** 3)
+    energy.zero()
+    capi.coulomb3d(atoms_buffer, work.atoms.shape[0], work.spacing, work.grid, energy)
+    volume = energy.download((work.grid, work.grid, work.grid))
+    energy.zero()
+    kernel_ms, wall_ms = capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
+    host = energy.download((work.grid, work.grid, work.grid))
+    energy.zero()
+    slices_ms = capi.coulomb3d_slices(atoms_buffer, work.atoms.shape[0], work.spacing, work.grid, energy)
+    fused = energy.download((work.grid, work.grid, work.grid))
+    for name, result, ms in (("host dz2", host, kernel_ms), ("device dz2", fused, slices_ms)):
+        worst, rms = oracle.normalised_error(result[points[:, 2], points[:, 1], points[:, 0]], values, norms)
+        difference = float(np.max(np.abs(result - volume)))
+        print(f"iterative {name}: {ms:.1f} ms, error max {worst:.2e} rms {rms:.2e}, |iterative - volume| {difference:.2e}",
+              flush=True)
+        assert worst <= PLAIN_BAR
+        assert difference <= SIDE_BY_SIDE
+    assert wall_ms >= kernel_ms > 0.0
+
+
+def test_full_size_volume_properties(capi):
+    """256^3 x 65,536 atoms: FP64 error at 4,096 sampled points, linearity and sign flip."""
+    work = workloads.volume3d()
+    points = workloads.sample_points(33, 4096, work.grid)
     atoms_buffer = capi.DeviceBuffer.from_array(work.atoms)
     energy = capi.DeviceBuffer(4 * work.grid ** 3)
     for form, bar in ((capi.PLAIN, PLAIN_BAR), (capi.COMPENSATED, COMPENSATED_BAR)):

@@ -52,6 +52,30 @@ def test_dynamic_run_moves_atoms_like_the_oracle_and_keeps_the_map_exact(capi, t
     assert results["UserData"]["workload"] == "dynamic"
 
 
+def test_dynamic_full_size_short_run(capi, tmp_path):
+    """The dynamic workload at its real size (192^3 grid, 131,072 atoms, sigma 0.05 A) for 20
+    steps, 8 of them MCMC tuning steps: the device jitter follows the oracle's restatement and
+    the final map meets the compensated bar against FP64 at 4,096 points."""
+    steps, budget, sigma = 20, 8, workloads.DYNAMIC_SIGMA
+    work = workloads.dynamic_initial()
+    out = capi.coulomb3d_dynamic(work.atoms, work.spacing, work.grid, steps, sigma, workloads.SEED_DYNAMIC,
+                                 searcher=capi.SEARCHER_MCMC, tuning_budget=budget, results_file=str(tmp_path / "dynamic_full"),
+                                 want_final=True)
+    print("dynamic full size: best", out["best_config"], "kernel ms", np.round(out["kernel_ms"], 2).tolist(), flush=True)
+    assert out["tested"] == budget
+    assert np.all(out["kernel_ms"] > 0.0)
+    expected = work.atoms
+    for step in range(steps):
+        expected = oracle.jitter(expected, workloads.SEED_DYNAMIC, step, sigma)
+    np.testing.assert_allclose(out["atoms"], expected, rtol=0, atol=1e-5 * steps)
+    np.testing.assert_array_equal(out["atoms"][:, 3], work.atoms[:, 3])
+    points = workloads.sample_points(79, 4096, work.grid)
+    values, norms = oracle.points_f64(out["atoms"], work.spacing, points)
+    error, rms = oracle.normalised_error(out["grid"][points[:, 2], points[:, 1], points[:, 0]], values, norms)
+    print(f"dynamic full size final-map error max {error:.2e} rms {rms:.2e}", flush=True)
+    assert error <= COMPENSATED_BAR
+
+
 def test_dynamic_production_phase_runs_the_fastest_tested_configuration(capi, tmp_path):
     budget = 8
     _, out, prefix = run_small(capi, tmp_path, capi.SEARCHER_DETERMINISTIC, budget, steps=12, atoms_count=2000, seed=79)
