@@ -765,7 +765,9 @@ int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize
             * static_cast<size_t>(shape.pointsPerThread) * 8;
         partials = runtime.GetScratch("coulomb3d.partials", bytes);
     }
-    void* counters = runtime.GetScratch("coulomb3d.counters", tiles * sizeof(unsigned int));
+    // One arrival counter per split tile.
+    void* counters = runtime.GetScratch("coulomb3d.counters", std::max<size_t>(1, tiles - static_cast<size_t>(splitFrom))
+        * sizeof(unsigned int));
     void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid, &partials, &counters,
         &splitFrom};
     CoulombRuntime::Launch(*kernel, grid, shape.block, arguments, hipStream, kernelMs);
