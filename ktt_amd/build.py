@@ -126,6 +126,35 @@ def build_core_tests() -> Path:
     return target
 
 
+SANITIZE_DIR = OBJ_DIR / "sanitize"
+SANITIZE_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined", "-g", "-O1"]
+
+
+def build_sanitized(jobs: int = 8) -> tuple[Path, Path]:
+    """Host-only AddressSanitizer + UBSan build of libktt_hip.so and ktt_core_tests into
+    ktt_amd/_build/sanitize/ (g++; the device code is compiled at run time by hipRTC and is not
+    instrumented). Run the core tests as ``ktt_amd/_build/sanitize/ktt_core_tests
+    tests/golden/coulomb3d_2080_space.json``; for the Python CPU tests, preload the sanitizer
+    runtime (``LD_PRELOAD=$(g++ -print-file-name=libasan.so)``, ``ASAN_OPTIONS=detect_leaks=0``)
+    and load this library in place of the product one."""
+    SANITIZE_DIR.mkdir(parents=True, exist_ok=True)
+    flags = [flag for flag in CXX_FLAGS if flag != "-O2"] + SANITIZE_FLAGS
+
+    def compile_one(source: Path) -> Path:
+        target = SANITIZE_DIR / (source.stem + ".o")
+        _run(["g++", *flags, "-c", str(source), "-o", str(target)])
+        return target
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        objects = list(pool.map(compile_one, sorted((CSRC / "src").glob("*.cpp"))))
+    library = SANITIZE_DIR / "libktt_hip.so"
+    _run(["g++", "-shared", *SANITIZE_FLAGS, "-o", str(library), *map(str, objects), *LINK_FLAGS])
+    program = SANITIZE_DIR / "ktt_core_tests"
+    _run(["g++", *flags, str(CSRC / "tests" / "CoreTests.cpp"), "-o", str(program), f"-L{SANITIZE_DIR}", "-lktt_hip",
+          f"-Wl,-rpath,{SANITIZE_DIR}", *LINK_FLAGS])
+    return library, program
+
+
 def check_kernels() -> None:
     library = ctypes.CDLL(str(LIBRARY))
     library.ktt_hip_compile_check.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
@@ -160,5 +189,9 @@ def build(verbose: bool = True) -> None:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    if len(sys.argv) > 1 and sys.argv[1] == "sanitize":
+        for path in build_sanitized():
+            print(path)
+    else:
+        build(verbose=True)
     sys.exit(0)

@@ -15,6 +15,7 @@
 
 #include "../../../include/ktt_hip.h"
 #include "Common.h"
+#include "CoulombBatchPlan.h"
 #include "CoulombRuntime.h"
 #include "HipEngine.h"
 #include "Json.h"
@@ -25,33 +26,10 @@ namespace ktt
 namespace
 {
 
-constexpr int kBatchThreads = 256;
-constexpr int kMaxBuckets = 5;
-
-// Layout shared with struct BatchSystem in kernels/CoulombBatched.hip.
-struct HostSystem
-{
-    int atomOffset;
-    int atomCount;
-    int gridSize;
-    int bucket;
-    float spacing;
-    int reserved;
-    long long gridOffset;
-};
-static_assert(sizeof(HostSystem) == 32, "BatchSystem layout");
-
-struct BucketShape
-{
-    int wgx;
-    int px;
-    int rows;
-    int tile;
-    int unroll = 1;  // unroll of the per-tile atom loop
-
-    int Span() const { return wgx * px; }
-    int RowsPerGroup() const { return (kBatchThreads / wgx) * rows; }
-};
+using batch::BucketShape;
+using batch::HostSystem;
+using batch::kBatchThreads;
+using batch::kMaxBuckets;
 
 // Defaults for buckets 0..4 (the kernel's own defaults; the host always passes them explicitly):
 // the fastest shapes of a bucket tuning run (MCMC) on the 1,024-system batch of grid sides 32, 48,
@@ -59,13 +37,16 @@ struct BucketShape
 const BucketShape kDefaultShapes[kMaxBuckets] = {{16, 2, 4, 64, 8}, {16, 3, 4, 64, 8}, {32, 2, 8, 64, 8}, {32, 3, 8, 128, 8},
     {64, 2, 8, 128, 8}};
 
-// Plain single-system kernel configuration of the per-system baseline, by grid side. Every
-// parameter is given, so that the baseline does not follow the defaults chosen for 256^3 grids:
-// atom tiles of 128 with the loop unrolled 8 times are the fastest of those measured on the
-// 1,024-system batch (unroll 16, the 256^3 default, is slower on these small grids).
+// Single-system kernel configuration of the per-system baseline, by grid side. Every parameter
+// is given, so that the baseline does not follow the defaults chosen for 256^3 grids: atom tiles
+// of 128 with the loop unrolled 8 times are the fastest of those measured on the 1,024-system
+// batch (unroll 16, the 256^3 default, is slower on these small grids). One atom block and, for
+// the compensated form, no tail split (TAIL_SPLIT=0): each system is one launch of whole tiles,
+// with no FP64 partials (a small system has fewer tiles than the GPU holds at once, so the
+// library default would split every one of them).
 std::string PlainConfigForSide(const int side)
 {
-    const std::string rest = ",ATOM_TILE=128,INNER_UNROLL_FACTOR=8,ATOM_BLOCKS=1";
+    const std::string rest = ",ATOM_TILE=128,INNER_UNROLL_FACTOR=8,ATOM_BLOCKS=1,TAIL_SPLIT=0";
     if (side % 128 == 0) return "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=2,ROWS_PER_THREAD=8" + rest;
     if (side % 96 == 0) return "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=8,POINTS_X=3,ROWS_PER_THREAD=8" + rest;
     if (side % 64 == 0) return "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=8,POINTS_X=2,ROWS_PER_THREAD=8" + rest;
@@ -243,14 +224,7 @@ CoulombBatch::CoulombBatch(const float* hostAtoms, const int* atomCounts, const 
     m_GridPoints = static_cast<size_t>(gridOffset);
 
     // Item capacity for the configured shapes; the tuner grows it for its own space.
-    size_t capacity = 0;
-    for (const auto& system : m_Systems)
-    {
-        const auto& shape = m_Shapes[static_cast<size_t>(system.bucket)];
-        const size_t g = static_cast<size_t>(system.gridSize);
-        capacity += ((g + shape.Span() - 1) / shape.Span()) * ((g * g + shape.RowsPerGroup() - 1) / shape.RowsPerGroup());
-    }
-    m_ItemCapacity = std::max<size_t>(capacity, 1);
+    m_ItemCapacity = std::max<size_t>(batch::CountWorkItems(m_Systems, m_Shapes), 1);
 
     CheckHipError(hipMalloc(reinterpret_cast<void**>(&m_Atoms), std::max<size_t>(static_cast<size_t>(atomOffset), 1) * 16), "hipMalloc(atoms)");
     CheckHipError(hipMalloc(&m_DeviceSystems, m_Systems.size() * sizeof(HostSystem)), "hipMalloc(systems)");
@@ -289,45 +263,7 @@ CoulombBatch::~CoulombBatch()
 
 std::vector<int> CoulombBatch::BuildItems(const std::vector<BucketShape>& shapes) const
 {
-    struct Item
-    {
-        long long cost;
-        int system;
-        int xBase;
-        int rowBase;
-    };
-    std::vector<Item> list;
-    for (size_t s = 0; s < m_Systems.size(); ++s)
-    {
-        const auto& system = m_Systems[s];
-        const auto& shape = shapes[static_cast<size_t>(system.bucket)];
-        const int g = system.gridSize;
-        const long long rows = static_cast<long long>(g) * g;
-        for (int xBase = 0; xBase < g; xBase += shape.Span())
-        {
-            for (long long rowBase = 0; rowBase < rows; rowBase += shape.RowsPerGroup())
-            {
-                const long long width = std::min<long long>(shape.Span(), g - xBase);
-                const long long height = std::min<long long>(shape.RowsPerGroup(), rows - rowBase);
-                // Cost: computed points (padded lanes included) times atom tiles.
-                const long long tiles = (system.atomCount + shape.tile - 1) / shape.tile;
-                list.push_back({static_cast<long long>(shape.Span()) * shape.RowsPerGroup() * tiles * shape.tile + width * height,
-                    static_cast<int>(s), xBase, static_cast<int>(rowBase)});
-            }
-        }
-    }
-    // Longest processing time first; ties keep a deterministic order.
-    std::stable_sort(list.begin(), list.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-    std::vector<int> flat;
-    flat.reserve(list.size() * 4);
-    for (const auto& item : list)
-    {
-        flat.push_back(item.system);
-        flat.push_back(item.xBase);
-        flat.push_back(item.rowBase);
-        flat.push_back(0);
-    }
-    return flat;
+    return batch::BuildWorkItems(m_Systems, shapes);
 }
 
 void CoulombBatch::EnsureItemCapacity(const size_t items)
@@ -367,10 +303,7 @@ size_t CoulombBatch::WorstCaseItems(const int bucket, const int minimumSpan, con
 
 void CoulombBatch::UploadItems(const std::vector<int>& items)
 {
-    if (items.size() / 4 > m_ItemCapacity)
-    {
-        throw KttException("work list of " + std::to_string(items.size() / 4) + " items exceeds the batch capacity");
-    }
+    batch::CheckItemCapacity(items, m_ItemCapacity);
     m_Items = items;
     if (!items.empty())
     {
@@ -740,6 +673,81 @@ static KernelResult TuneBucket(CoulombBatch& batch, const int bucket, const int 
     bestMs = static_cast<double>(winner->GetKernelDuration()) / 1.0e6;
     return *winner;
 }
+
+
+namespace batch
+{
+
+size_t CountWorkItems(const std::vector<HostSystem>& systems, const std::vector<BucketShape>& shapes)
+{
+    size_t total = 0;
+    for (const auto& system : systems)
+    {
+        const auto& shape = shapes.at(static_cast<size_t>(system.bucket));
+        const size_t g = static_cast<size_t>(system.gridSize);
+        total += ((g + shape.Span() - 1) / shape.Span()) * ((g * g + shape.RowsPerGroup() - 1) / shape.RowsPerGroup());
+    }
+    return total;
+}
+
+std::vector<int> BuildWorkItems(const std::vector<HostSystem>& systems, const std::vector<BucketShape>& shapes)
+{
+    struct Item
+    {
+        long long cost;
+        int system;
+        int xBase;
+        int rowBase;
+    };
+    std::vector<Item> list;
+    list.reserve(CountWorkItems(systems, shapes));
+    for (size_t s = 0; s < systems.size(); ++s)
+    {
+        const auto& system = systems[s];
+        const auto& shape = shapes.at(static_cast<size_t>(system.bucket));
+        const int g = system.gridSize;
+        const long long rows = static_cast<long long>(g) * g;
+        // Cost: computed points (padded lanes included) times staged atoms, plus the stored points.
+        const long long tiles = (system.atomCount + shape.tile - 1) / shape.tile;
+        for (int xBase = 0; xBase < g; xBase += shape.Span())
+        {
+            for (long long rowBase = 0; rowBase < rows; rowBase += shape.RowsPerGroup())
+            {
+                const long long width = std::min<long long>(shape.Span(), g - xBase);
+                const long long height = std::min<long long>(shape.RowsPerGroup(), rows - rowBase);
+                list.push_back({static_cast<long long>(shape.Span()) * shape.RowsPerGroup() * tiles * shape.tile + width * height,
+                    static_cast<int>(s), xBase, static_cast<int>(rowBase)});
+            }
+        }
+    }
+    // Longest processing time first; ties keep a deterministic order.
+    std::stable_sort(list.begin(), list.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+    std::vector<int> flat;
+    flat.reserve(list.size() * 4);
+    for (const auto& item : list)
+    {
+        flat.push_back(item.system);
+        flat.push_back(item.xBase);
+        flat.push_back(item.rowBase);
+        flat.push_back(0);
+    }
+    return flat;
+}
+
+void CheckItemCapacity(const std::vector<int>& flat, const size_t capacity)
+{
+    if (flat.size() % 4 != 0)
+    {
+        throw KttException("work list of " + std::to_string(flat.size()) + " ints is not a list of int4 records");
+    }
+    if (flat.size() / 4 > capacity)
+    {
+        throw KttException("work list of " + std::to_string(flat.size() / 4) + " items exceeds the batch capacity of "
+            + std::to_string(capacity));
+    }
+}
+
+} // namespace batch
 
 } // namespace ktt
 

@@ -1,5 +1,6 @@
 #include <hip/hiprtc.h>
 
+#include <dlfcn.h>
 #include <unistd.h>
 
 #include <algorithm>
@@ -141,17 +142,48 @@ std::string IncludeFingerprint(const std::vector<std::string>& options)
     return fingerprint;
 }
 
+// The compiler's identity: hipRTC's version and the real path, size and modification time of the
+// hipRTC library and of the code-object manager beside it (which holds the compiler proper), so
+// that a patch release with a changed compiler and the same version number misses the cache.
+std::string CompilerIdentity()
+{
+    static const std::string identity = []()
+    {
+        int major = 0;
+        int minor = 0;
+        hiprtcVersion(&major, &minor);
+        std::string text = "hiprtc " + std::to_string(major) + "." + std::to_string(minor);
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void*>(&hiprtcVersion), &info) == 0 || info.dli_fname == nullptr)
+        {
+            return text;
+        }
+        const std::filesystem::path library = info.dli_fname;
+        for (const auto& path : {library, library.parent_path() / "libamd_comgr.so"})
+        {
+            std::error_code error;
+            const auto real = std::filesystem::canonical(path, error);
+            if (error)
+            {
+                continue;
+            }
+            const auto size = std::filesystem::file_size(real, error);
+            const auto time = std::filesystem::last_write_time(real, error).time_since_epoch().count();
+            text += ";" + real.string() + ":" + std::to_string(size) + ":" + std::to_string(static_cast<long long>(time));
+        }
+        return text;
+    }();
+    return identity;
+}
+
 std::string CacheKey(const std::string& source, const std::string& kernelName, const std::vector<std::string>& options)
 {
-    int major = 0;
-    int minor = 0;
-    hiprtcVersion(&major, &minor);
     std::string material = source + '\0' + kernelName + '\0';
     for (const auto& option : options)
     {
         material += option + '\0';
     }
-    material += std::to_string(major) + "." + std::to_string(minor) + '\0' + IncludeFingerprint(options);
+    material += CompilerIdentity() + '\0' + IncludeFingerprint(options);
     char key[40];
     std::snprintf(key, sizeof(key), "%016llx%016llx", static_cast<unsigned long long>(Fnv1a(material, 1469598103934665603ull)),
         static_cast<unsigned long long>(Fnv1a(material, 0x9E3779B97F4A7C15ull)));

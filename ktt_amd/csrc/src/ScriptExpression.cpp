@@ -1,8 +1,10 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <limits>
+#include <string>
 
 #include <KttException.h>
 
@@ -497,11 +499,66 @@ bool BothIntegral(const ScriptValue& a, const ScriptValue& b)
     return a.GetKind() != ScriptValue::Kind::Float && b.GetKind() != ScriptValue::Kind::Float;
 }
 
+// Python integers are unbounded; this evaluator keeps int64 and refuses a result outside it
+// rather than wrapping.
+int64_t CheckedAdd(const int64_t a, const int64_t b)
+{
+    int64_t result = 0;
+    if (__builtin_add_overflow(a, b, &result))
+    {
+        throw KttException("Integer overflow in " + std::to_string(a) + " + " + std::to_string(b));
+    }
+    return result;
+}
+
+int64_t CheckedSubtract(const int64_t a, const int64_t b)
+{
+    int64_t result = 0;
+    if (__builtin_sub_overflow(a, b, &result))
+    {
+        throw KttException("Integer overflow in " + std::to_string(a) + " - " + std::to_string(b));
+    }
+    return result;
+}
+
+int64_t CheckedMultiply(const int64_t a, const int64_t b)
+{
+    int64_t result = 0;
+    if (__builtin_mul_overflow(a, b, &result))
+    {
+        throw KttException("Integer overflow in " + std::to_string(a) + " * " + std::to_string(b));
+    }
+    return result;
+}
+
+// base ** exponent for exponent >= 0 by square-and-multiply, every step checked.
+int64_t CheckedPower(int64_t base, int64_t exponent)
+{
+    int64_t result = 1;
+    while (exponent > 0)
+    {
+        if (exponent & 1)
+        {
+            result = CheckedMultiply(result, base);
+        }
+        exponent >>= 1;
+        if (exponent > 0)
+        {
+            base = CheckedMultiply(base, base);
+        }
+    }
+    return result;
+}
+
 int64_t FloorDivide(const int64_t a, const int64_t b)
 {
     if (b == 0)
     {
         throw KttException("Integer division by zero");
+    }
+    if (a == INT64_MIN && b == -1)
+    {
+        throw KttException("Integer overflow in " + std::to_string(a) + " // -1");
     }
     int64_t quotient = a / b;
     if ((a % b != 0) && ((a < 0) != (b < 0)))
@@ -574,6 +631,10 @@ ScriptValue Arithmetic(const std::string& op, const ScriptValue& a, const Script
         {
             const int64_t x = a.ToInt();
             const int64_t y = b.ToInt();
+            if (op == "%" && y == -1)
+            {
+                return ScriptValue::Integer(0);
+            }
             const int64_t q = FloorDivide(x, y);
             return ScriptValue::Integer(op == "//" ? q : x - q * y);
         }
@@ -586,12 +647,7 @@ ScriptValue Arithmetic(const std::string& op, const ScriptValue& a, const Script
     {
         if (integral && b.ToInt() >= 0)
         {
-            int64_t result = 1;
-            for (int64_t k = 0; k < b.ToInt(); ++k)
-            {
-                result *= a.ToInt();
-            }
-            return ScriptValue::Integer(result);
+            return ScriptValue::Integer(CheckedPower(a.ToInt(), b.ToInt()));
         }
         return ScriptValue::Real(std::pow(a.ToDouble(), b.ToDouble()));
     }
@@ -601,13 +657,13 @@ ScriptValue Arithmetic(const std::string& op, const ScriptValue& a, const Script
         const int64_t y = b.ToInt();
         if (op == "+")
         {
-            return ScriptValue::Integer(x + y);
+            return ScriptValue::Integer(CheckedAdd(x, y));
         }
         if (op == "-")
         {
-            return ScriptValue::Integer(x - y);
+            return ScriptValue::Integer(CheckedSubtract(x, y));
         }
-        return ScriptValue::Integer(x * y);
+        return ScriptValue::Integer(CheckedMultiply(x, y));
     }
     const double x = a.ToDouble();
     const double y = b.ToDouble();
@@ -727,7 +783,8 @@ ScriptValue Evaluate(const Node& node, const ScriptExpression::Lookup& lookup)
         {
             return operand.GetKind() == ScriptValue::Kind::Bool ? ScriptValue::Integer(operand.ToInt()) : operand;
         }
-        return operand.GetKind() == ScriptValue::Kind::Float ? ScriptValue::Real(-operand.ToDouble()) : ScriptValue::Integer(-operand.ToInt());
+        return operand.GetKind() == ScriptValue::Kind::Float ? ScriptValue::Real(-operand.ToDouble())
+                                                             : ScriptValue::Integer(CheckedSubtract(0, operand.ToInt()));
     }
     case Node::Type::Binary:
         return Arithmetic(node.text, Evaluate(*node.children[0], lookup), Evaluate(*node.children[1], lookup));

@@ -5,7 +5,9 @@
 //
 // Usage: ktt_core_tests <path to tests/golden/coulomb3d_2080_space.json>
 // Exit status 0 when every check passes; each failure is printed with its line.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -14,6 +16,7 @@
 #include <set>
 #include <sstream>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include <Ktt.h>
@@ -24,6 +27,7 @@
 #include "KernelArgument.h"
 #include "TunerCore.h"
 #include "Exploration.h"
+#include "CoulombBatchPlan.h"
 
 using namespace ktt;
 
@@ -406,6 +410,118 @@ void TestProfileSearcherAnalysis()
 
 } // namespace
 
+/** The batch work list (CoulombBatchPlan.h) on a 1,024-system batch shaped like the benchmark's
+ *  (atom counts log-uniform in [200, 20000], sides 32/48/64/96/128, the built-in bucket shapes):
+ *  the item count is the tile count the device list is sized for, every (system, tile) appears
+ *  exactly once with in-range origins, the order is costliest first, and a list one item longer
+ *  than the capacity is refused. A wrong count here is what once let the persistent kernel read
+ *  past its list. */
+void TestBatchWorkList()
+{
+    using namespace ktt::batch;
+    const int sides[5] = {32, 48, 64, 96, 128};
+    const std::vector<BucketShape> shapes = {{16, 2, 4, 64, 8}, {16, 3, 4, 64, 8}, {32, 2, 8, 64, 8}, {32, 3, 8, 128, 8},
+        {64, 2, 8, 128, 8}};
+    std::vector<HostSystem> systems;
+    uint64_t state = 12345;
+    const auto next = [&state]()
+    {
+        state = state * 6364136223846793005ull + 1442695040888963407ull;
+        return static_cast<double>(state >> 11) * (1.0 / 9007199254740992.0);
+    };
+    int atomOffset = 0;
+    long long gridOffset = 0;
+    for (int s = 0; s < 1024; ++s)
+    {
+        HostSystem system{};
+        system.bucket = static_cast<int>(next() * 5.0) % 5;
+        system.gridSize = sides[system.bucket];
+        system.atomCount = static_cast<int>(std::lround(std::exp(std::log(200.0) + next() * (std::log(20000.0) - std::log(200.0)))));
+        system.atomOffset = atomOffset;
+        system.spacing = 0.5f;
+        system.gridOffset = gridOffset;
+        atomOffset += system.atomCount;
+        gridOffset += static_cast<long long>(system.gridSize) * system.gridSize * system.gridSize;
+        systems.push_back(system);
+    }
+    // A system with no atoms and one whose side is not a multiple of its bucket's span.
+    systems.push_back({atomOffset, 0, 32, 0, 0.5f, 0, gridOffset});
+    systems.push_back({atomOffset, 7, 40, 1, 0.5f, 0, gridOffset + 32 * 32 * 32});
+
+    const std::vector<int> flat = BuildWorkItems(systems, shapes);
+    const size_t count = CountWorkItems(systems, shapes);
+    CHECK(flat.size() == 4 * count);
+    size_t expected = 0;
+    std::set<std::tuple<int, int, int>> seen;
+    bool inRange = true;
+    bool ordered = true;
+    long long previousCost = -1;
+    for (size_t k = 0; k < flat.size() / 4; ++k)
+    {
+        const int s = flat[4 * k];
+        const int x = flat[4 * k + 1];
+        const int row = flat[4 * k + 2];
+        if (s < 0 || s >= static_cast<int>(systems.size()) || flat[4 * k + 3] != 0)
+        {
+            inRange = false;
+            continue;
+        }
+        const HostSystem& system = systems[static_cast<size_t>(s)];
+        const BucketShape& shape = shapes[static_cast<size_t>(system.bucket)];
+        const long long rows = static_cast<long long>(system.gridSize) * system.gridSize;
+        inRange = inRange && x >= 0 && x < system.gridSize && x % shape.Span() == 0 && row >= 0 && row < rows
+            && row % shape.RowsPerGroup() == 0;
+        seen.insert({s, x, row});
+        const long long width = std::min<long long>(shape.Span(), system.gridSize - x);
+        const long long height = std::min<long long>(shape.RowsPerGroup(), rows - row);
+        const long long cost = static_cast<long long>(shape.Span()) * shape.RowsPerGroup()
+            * ((system.atomCount + shape.tile - 1) / shape.tile) * shape.tile + width * height;
+        ordered = ordered && (previousCost < 0 || cost <= previousCost);
+        previousCost = cost;
+    }
+    for (const auto& system : systems)
+    {
+        const BucketShape& shape = shapes[static_cast<size_t>(system.bucket)];
+        const size_t g = static_cast<size_t>(system.gridSize);
+        expected += ((g + shape.Span() - 1) / shape.Span()) * ((g * g + shape.RowsPerGroup() - 1) / shape.RowsPerGroup());
+    }
+    CHECK(inRange);
+    CHECK(ordered);
+    CHECK(count == expected);
+    CHECK(seen.size() == count);  // no (system, tile) twice, so all `expected` tiles appear
+    CHECK(count > 100000);
+    bool accepted = true;
+    try
+    {
+        CheckItemCapacity(flat, count);
+    }
+    catch (const KttException&)
+    {
+        accepted = false;
+    }
+    CHECK(accepted);
+    bool refused = false;
+    try
+    {
+        CheckItemCapacity(flat, count - 1);
+    }
+    catch (const KttException&)
+    {
+        refused = true;
+    }
+    CHECK(refused);
+    bool ragged = false;
+    try
+    {
+        CheckItemCapacity(std::vector<int>(flat.begin(), flat.end() - 1), count);
+    }
+    catch (const KttException&)
+    {
+        ragged = true;
+    }
+    CHECK(ragged);
+}
+
 int main(int argc, char** argv)
 {
     if (argc != 2)
@@ -423,6 +539,7 @@ int main(int argc, char** argv)
     TestStopConditions(results);
     TestJsonRoundTrip(results);
     TestProfileSearcherAnalysis();
+    TestBatchWorkList();
     std::printf("%d checks, %d failures\n", g_Checks, g_Failures);
     return g_Failures == 0 ? 0 : 1;
 }

@@ -49,6 +49,18 @@ def test_generator_errors_are_reported():
             capi.script_generate(bad, 4)
 
 
+def test_integer_results_beyond_int64_are_refused_not_wrapped():
+    """Python's integers are unbounded; the evaluator keeps int64 and reports a result outside it
+    instead of wrapping. Powers are square-and-multiply, so a huge exponent ends at once."""
+    np.testing.assert_array_equal(capi.script_generate("2 ** 62 + (2 ** 62 - 1)", 1), [float(2 ** 62 + (2 ** 62 - 1))])
+    np.testing.assert_array_equal(capi.script_generate("(-2) ** 63 // 1", 1), [float((-2) ** 63)])
+    np.testing.assert_array_equal(capi.script_generate("3 ** 39 * (i + 1)", 2), [float(3 ** 39), float(2 * 3 ** 39)])
+    for bad in ("2 ** 64", "2 ** 10 ** 12", "2 ** 62 * 2", "-(2 ** 62) - 2 ** 62 - 1", "(-2) ** 63 // -1",
+                "-((-2) ** 63)", "4611686018427387904 + 4611686018427387904"):
+        with pytest.raises(capi.KttHipError, match="overflow"):
+            capi.script_generate(bad, 1)
+
+
 def reference_space_script(path, open_cl=False, conditions=True):
     parameters = [
         ("WORK_GROUP_SIZE_X", "[16, 32]"), ("WORK_GROUP_SIZE_Y", "[1, 2, 4, 8]"), ("WORK_GROUP_SIZE_Z", "[1]"),
