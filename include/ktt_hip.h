@@ -91,6 +91,17 @@ int ktt_hip_event_elapsed_ms(void* start, void* end, float* milliseconds);
 int ktt_coulomb3d(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                   int form, const char* config, void* stream, float* kernel_ms);
 
+/* ktt_coulomb3d in the compensated form with the default configuration, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). The kernels are
+ * compiled ahead of time (libktt_hip_noalloc.so, loaded on the first call) and their scratch
+ * (the tail split's FP64-merged partials and arrival counters) is a static device array of
+ * 64 MiB + 64 KiB per device; a launch needing more is rejected. The first call on a device
+ * sets up (symbol address, occupancy): make it outside graph capture. Calls that overlap on
+ * one device race on the shared scratch. grid_size in [1, 1023]; atoms 16-byte aligned. */
+int ktt_coulomb3d_compensated_noalloc(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
+                                      void* stream);
+
 /* The one-point-per-thread validation kernel (Examples/CoulombSum3d/CoulombSum3dReference.cu:1-28). */
 int ktt_coulomb3d_reference(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                             void* stream, float* kernel_ms);

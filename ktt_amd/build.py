@@ -28,8 +28,12 @@ KERNELS = PACKAGE / "kernels"
 OBJ_DIR = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libktt_hip.so"
 COUNTERS_LIBRARY = PACKAGE / "libktt_hip_counters.so"
+NOALLOC_LIBRARY = PACKAGE / "libktt_hip_noalloc.so"
 BIN_DIR = PACKAGE / "bin"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+ARCH = "gfx950"
+# Device-code flags of every hipcc build: packed FP32 instructions off (as for hipRTC).
+HIP_DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 CXX_FLAGS = [
     "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
@@ -101,6 +105,17 @@ def build_counters() -> Path:
           f"-Wl,-rpath,{ROCM / 'lib'}", "-pthread"])
     os.replace(temporary, COUNTERS_LIBRARY)
     return COUNTERS_LIBRARY
+
+
+def build_noalloc() -> Path:
+    """libktt_hip_noalloc.so: the precompiled compensated kernels behind
+    ktt_coulomb3d_compensated_noalloc, with their scratch in a static device array (hipcc for
+    gfx950; libktt_hip.so loads it on first use)."""
+    temporary = NOALLOC_LIBRARY.with_suffix(".so.tmp")
+    _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", *HIP_DEVICE_FLAGS,
+          f"-I{KERNELS}", str(CSRC / "noalloc" / "CoulombNoAlloc.hip"), "-o", str(temporary)])
+    os.replace(temporary, NOALLOC_LIBRARY)
+    return NOALLOC_LIBRARY
 
 
 def build_examples() -> list[Path]:
@@ -179,6 +194,9 @@ def build(verbose: bool = True) -> None:
         print("[ktt_amd.build] compiling libktt_hip.so", flush=True)
     build_library()
     build_counters()
+    if verbose:
+        print("[ktt_amd.build] compiling libktt_hip_noalloc.so (hipcc, gfx950)", flush=True)
+    build_noalloc()
     if verbose:
         print("[ktt_amd.build] compiling examples", flush=True)
     build_examples()

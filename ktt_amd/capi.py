@@ -48,6 +48,7 @@ SIGNATURES = {
     "ktt_hip_event_elapsed_ms": (_c.c_int, [_vp, _vp, _fp]),
     "ktt_coulomb3d": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_int, _c.c_char_p, _vp, _fp]),
     "ktt_coulomb3d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _vp, _fp]),
+    "ktt_coulomb3d_compensated_noalloc": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _vp]),
     "ktt_coulomb2d": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _c.c_char_p, _vp, _fp]),
     "ktt_coulomb2d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _vp, _fp]),
     "ktt_coulomb3d_iterative": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_char_p, _vp, _fp, _fp]),
@@ -232,6 +233,13 @@ def coulomb3d(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, ener
     _check(library().ktt_coulomb3d(atoms.ptr, n_atoms, spacing, grid, energy.ptr, form, _config(config), stream,
                                    ctypes.byref(ms) if timed else None), "ktt_coulomb3d")
     return ms.value if timed else None
+
+
+def coulomb3d_compensated_noalloc(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, energy: DeviceBuffer,
+                                  stream=None) -> None:
+    """The compensated 3D sum through the entry point that allocates nothing (asynchronous)."""
+    _check(library().ktt_coulomb3d_compensated_noalloc(atoms.ptr, n_atoms, spacing, grid, energy.ptr, stream),
+           "ktt_coulomb3d_compensated_noalloc")
 
 
 def coulomb3d_reference(atoms: DeviceBuffer, n_atoms: int, spacing: float, grid: int, energy: DeviceBuffer, stream=None,

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 
 #include <KttException.h>
@@ -771,6 +772,48 @@ int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize
     void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid, &partials, &counters,
         &splitFrom};
     CoulombRuntime::Launch(*kernel, grid, shape.block, arguments, hipStream, kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
+// The no-allocation variant lives in libktt_hip_noalloc.so (built by hipcc, with its scratch in a
+// static device array); it is loaded on the first call, so that processes that never use it never
+// register its code object.
+using NoAllocEntry = int (*)(const float*, int, float, int, float*, void*, char*, size_t);
+
+static NoAllocEntry LoadNoAllocEntry()
+{
+    static std::mutex mutex;
+    static NoAllocEntry entry = nullptr;
+    std::lock_guard<std::mutex> lock(mutex);
+    if (entry == nullptr)
+    {
+        const std::string path = LibraryDirectory() + "/libktt_hip_noalloc.so";
+        void* library = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (library == nullptr)
+        {
+            const char* reason = dlerror();
+            throw KttException("cannot load " + path + (reason != nullptr ? std::string(": ") + reason : std::string()));
+        }
+        entry = reinterpret_cast<NoAllocEntry>(dlsym(library, "ktt_noalloc_coulomb3d_compensated"));
+        if (entry == nullptr)
+        {
+            throw KttException(path + " does not export ktt_noalloc_coulomb3d_compensated");
+        }
+    }
+    return entry;
+}
+
+int ktt_coulomb3d_compensated_noalloc(const float* atoms, int atomCount, float spacing, int gridSize, float* energyGrid,
+    void* stream)
+{
+    KTT_C_TRY
+    const NoAllocEntry entry = LoadNoAllocEntry();
+    char message[512] = {};
+    if (entry(atoms, atomCount, spacing, gridSize, energyGrid, stream, message, sizeof(message)) != 0)
+    {
+        return SetLastError(message);
+    }
     return 0;
     KTT_C_CATCH
 }

@@ -79,3 +79,20 @@ def test_unknown_form_is_rejected():
     status = library.ktt_coulomb3d(dummy, 0, 0.5, 16, dummy, 7, None, None, None)
     assert status != 0
     assert "form" in library.ktt_hip_last_error().decode()
+
+
+def test_no_alloc_library_loads_and_checks_input_before_the_device():
+    """ktt_coulomb3d_compensated_noalloc loads libktt_hip_noalloc.so (built by hipcc) on its first
+    call; input errors are reported before any device work, and without a device the set-up
+    reports that instead of launching."""
+    assert native_build.NOALLOC_LIBRARY.exists()
+    exported = ctypes.CDLL(str(native_build.NOALLOC_LIBRARY))
+    assert hasattr(exported, "ktt_noalloc_coulomb3d_compensated")
+    library = capi.library()
+    dummy = ctypes.c_void_p(16)
+    assert library.ktt_coulomb3d_compensated_noalloc(dummy, 10, 0.5, 1024, dummy, None) != 0
+    assert "grid size" in library.ktt_hip_last_error().decode()
+    assert library.ktt_coulomb3d_compensated_noalloc(None, 10, 0.5, 16, dummy, None) != 0
+    assert "null" in library.ktt_hip_last_error().decode()
+    assert library.ktt_coulomb3d_compensated_noalloc(ctypes.c_void_p(20), 10, 0.5, 16, dummy, None) != 0
+    assert "aligned" in library.ktt_hip_last_error().decode()
