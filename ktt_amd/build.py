@@ -133,11 +133,14 @@ def build_examples() -> list[Path]:
 
 
 def build_core_tests() -> Path:
-    """Host-only test program of the ktt core (tests/test_core.py runs it)."""
+    """Host-only test program of the ktt core (tests/test_core.py runs it), and the GPU test
+    program of symbol arguments (tests/test_gpu_symbol.py runs it)."""
     BIN_DIR.mkdir(exist_ok=True)
     target = BIN_DIR / "ktt_core_tests"
     _run(["g++", *CXX_FLAGS, str(CSRC / "tests" / "CoreTests.cpp"), "-o", str(target), f"-L{PACKAGE}", "-lktt_hip",
           f"-Wl,-rpath,{PACKAGE}", *LINK_FLAGS])
+    _run(["g++", *CXX_FLAGS, str(CSRC / "tests" / "SymbolArgumentTest.cpp"), "-o", str(BIN_DIR / "ktt_symbol_test"),
+          f"-L{PACKAGE}", "-lktt_hip", f"-Wl,-rpath,{PACKAGE}", *LINK_FLAGS])
     return target
 
 

@@ -83,6 +83,11 @@ public:
     ArgumentId AddArgumentScalar(const void* data, const size_t dataSize, const ArgumentId& customId = "");
     template <typename T>
     ArgumentId AddArgumentLocal(const size_t localMemorySize, const ArgumentId& customId = "");
+    /** Value copied into the kernel's __device__ or __constant__ variable symbolName when a
+     *  configuration's kernel is loaded (Source/Tuner.h:501-510); the variable takes no kernel
+     *  parameter slot. On HIP the symbol name is required. */
+    template <typename T>
+    ArgumentId AddArgumentSymbol(const T& data, const ArgumentId& customId = "", const std::string& symbolName = "");
     void RemoveArgument(const ArgumentId& id);
     void SetReadOnlyArgumentCache(const bool flag);
 
@@ -171,6 +176,8 @@ private:
     ArgumentId RegisterArgument(const size_t elementSize, const ArgumentDataType dataType, const ArgumentMemoryLocation location,
         const ArgumentAccessType access, const ArgumentMemoryType kind, const ArgumentManagementType management, const DataSource source,
         void* data, const size_t bytes, const ArgumentId& name);
+    ArgumentId RegisterSymbolArgument(const size_t elementSize, const ArgumentDataType dataType, const void* data, const size_t bytes,
+        const ArgumentId& name, const std::string& symbolName);
     void AddParameterValues(const KernelId id, const std::string& name, const std::vector<ParameterValue>& values,
         const std::string& group);
 

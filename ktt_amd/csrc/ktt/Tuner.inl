@@ -77,6 +77,16 @@ ArgumentId Tuner::AddArgumentLocal(const size_t localMemorySize, const ArgumentI
 }
 
 template <typename T>
+ArgumentId Tuner::AddArgumentSymbol(const T& data, const ArgumentId& customId, const std::string& symbolName)
+{
+    static_assert(std::is_trivially_copyable_v<T>, "Argument data type must be trivially copyable");
+    static_assert(!std::is_reference_v<T> && !std::is_pointer_v<T> && !std::is_null_pointer_v<T>,
+        "Pointer and reference argument data types are not supported");
+    static_assert(!std::is_same_v<std::remove_cv_t<T>, bool>, "Bool argument data type is not supported");
+    return RegisterSymbolArgument(sizeof(T), ElementTypeOf<T>(), &data, sizeof(T), customId, symbolName);
+}
+
+template <typename T>
 ArgumentDataType Tuner::ElementTypeOf()
 {
     // Fixed-width types map to their KTT data types; anything else is opaque (Custom).

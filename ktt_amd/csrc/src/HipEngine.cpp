@@ -225,36 +225,66 @@ void WriteFileAtomically(const std::string& path, const char* data, const size_t
 std::vector<char> HipCompiler::Compile(const std::string& source, const std::string& programName, const std::string& kernelName,
     const std::vector<std::string>& options, std::string& loweredName, std::string* log)
 {
+    std::vector<std::string> unused;
+    return Compile(source, programName, kernelName, options, {}, loweredName, unused, log);
+}
+
+std::vector<char> HipCompiler::Compile(const std::string& source, const std::string& programName, const std::string& kernelName,
+    const std::vector<std::string>& options, const std::vector<std::string>& symbols, std::string& loweredName,
+    std::vector<std::string>& loweredSymbols, std::string* log)
+{
+    // The cache's .name file holds the kernel's lowered name, then one line per symbol.
     const char* cacheDirectory = std::getenv("KTT_HIP_CACHE_DIR");
     std::string cachePath;
     if (cacheDirectory != nullptr && cacheDirectory[0] != '\0')
     {
         std::error_code error;
         std::filesystem::create_directories(cacheDirectory, error);
-        cachePath = std::string(cacheDirectory) + "/" + CacheKey(source, kernelName, options);
-        std::string code;
-        std::string name;
-        if (ReadFile(cachePath + ".name", name) && ReadFile(cachePath + ".co", code) && !code.empty() && !name.empty())
+        std::vector<std::string> keyOptions = options;
+        for (const auto& symbol : symbols)
         {
-            loweredName = name;
-            if (log != nullptr)
+            keyOptions.push_back("symbol:" + symbol);
+        }
+        cachePath = std::string(cacheDirectory) + "/" + CacheKey(source, kernelName, keyOptions);
+        std::string code;
+        std::string names;
+        if (ReadFile(cachePath + ".name", names) && ReadFile(cachePath + ".co", code) && !code.empty() && !names.empty())
+        {
+            std::vector<std::string> lines;
+            std::stringstream stream(names);
+            for (std::string line; std::getline(stream, line);)
             {
-                log->clear();
+                lines.push_back(line);
             }
-            return std::vector<char>(code.begin(), code.end());
+            if (lines.size() == symbols.size() + 1 && !lines[0].empty())
+            {
+                loweredName = lines[0];
+                loweredSymbols.assign(lines.begin() + 1, lines.end());
+                if (log != nullptr)
+                {
+                    log->clear();
+                }
+                return std::vector<char>(code.begin(), code.end());
+            }
         }
     }
-    std::vector<char> code = CompileWithRtc(source, programName, kernelName, options, loweredName, log);
+    std::vector<char> code = CompileWithRtc(source, programName, kernelName, options, loweredName, log, symbols, &loweredSymbols);
     if (!cachePath.empty())
     {
+        std::string names = loweredName;
+        for (const auto& symbol : loweredSymbols)
+        {
+            names += "\n" + symbol;
+        }
         WriteFileAtomically(cachePath + ".co", code.data(), code.size());
-        WriteFileAtomically(cachePath + ".name", loweredName.data(), loweredName.size());
+        WriteFileAtomically(cachePath + ".name", names.data(), names.size());
     }
     return code;
 }
 
 std::vector<char> HipCompiler::CompileWithRtc(const std::string& source, const std::string& programName, const std::string& kernelName,
-    const std::vector<std::string>& options, std::string& loweredName, std::string* log)
+    const std::vector<std::string>& options, std::string& loweredName, std::string* log, const std::vector<std::string>& symbols,
+    std::vector<std::string>* loweredSymbols)
 {
     hiprtcProgram program;
     CheckRtcError(hiprtcCreateProgram(&program, source.c_str(), programName.c_str(), 0, nullptr, nullptr), "hiprtcCreateProgram");
@@ -269,6 +299,13 @@ std::vector<char> HipCompiler::CompileWithRtc(const std::string& source, const s
     if (templated)
     {
         CheckRtcError(hiprtcAddNameExpression(program, kernelName.c_str()), "hiprtcAddNameExpression");
+    }
+    // Global variables of symbol arguments are looked up by their lowered names after compilation
+    // (as Source/ComputeEngine/Cuda/CudaProgram.cpp:39-41 does with NVRTC); "&name" is the name
+    // expression of a variable.
+    for (const auto& symbol : symbols)
+    {
+        CheckRtcError(hiprtcAddNameExpression(program, ("&" + symbol).c_str()), "hiprtcAddNameExpression(" + symbol + ")");
     }
 
     std::vector<const char*> rawOptions;
@@ -306,6 +343,16 @@ std::vector<char> HipCompiler::CompileWithRtc(const std::string& source, const s
     else
     {
         loweredName = kernelName;
+    }
+    if (loweredSymbols != nullptr)
+    {
+        loweredSymbols->clear();
+        for (const auto& symbol : symbols)
+        {
+            const char* lowered = nullptr;
+            CheckRtcError(hiprtcGetLoweredName(program, ("&" + symbol).c_str(), &lowered), "hiprtcGetLoweredName(" + symbol + ")");
+            loweredSymbols->push_back(lowered);
+        }
     }
 
     size_t codeSize = 0;
@@ -361,6 +408,22 @@ HipKernel::~HipKernel()
     {
         IgnoreHipError(hipModuleUnload(m_Module));
     }
+}
+
+void HipKernel::FillSymbol(const std::string& loweredName, const void* data, const size_t bytes) const
+{
+    hipDeviceptr_t address = nullptr;
+    size_t size = 0;
+    const hipError_t status = hipModuleGetGlobal(&address, &size, m_Module, loweredName.c_str());
+    if (status != hipSuccess || address == nullptr)
+    {
+        throw KttException("Symbol " + loweredName + " was not found in the module of " + m_Name);
+    }
+    if (size < bytes)
+    {
+        throw KttException("Symbol " + loweredName + " holds " + std::to_string(size) + " bytes, its argument " + std::to_string(bytes));
+    }
+    CheckHipError(hipMemcpyHtoD(address, const_cast<void*>(data), bytes), "hipMemcpyHtoD(symbol " + loweredName + ")");
 }
 
 // ---------------------------------------------------------------- HipEngine
@@ -500,13 +563,30 @@ std::shared_ptr<HipKernel> HipEngine::LoadKernel(const KernelComputeData& data, 
         }
     }
 
+    // Symbol arguments are copied into their variables once per loaded module, as the reference
+    // does when it creates a kernel (Source/ComputeEngine/Cuda/CudaProgram.cpp:67-78).
+    std::vector<const KernelArgument*> symbolArguments;
+    std::vector<std::string> symbolNames;
+    for (const auto* argument : data.GetArguments())
+    {
+        if (argument->GetMemoryType() == ArgumentMemoryType::Symbol)
+        {
+            symbolArguments.push_back(argument);
+            symbolNames.push_back(argument->GetSymbolName());
+        }
+    }
     Timer timer;
     timer.Start();
     std::string loweredName;
+    std::vector<std::string> loweredSymbols;
     const auto code = HipCompiler::Compile(data.GetDefaultSource(), data.GetName() + ".hip", data.GetTemplatedName(),
-        BuildOptions(data), loweredName);
+        BuildOptions(data), symbolNames, loweredName, loweredSymbols);
     timer.Stop();
     auto kernel = std::make_shared<HipKernel>(data.GetName(), code, loweredName, timer.GetElapsedTime());
+    for (size_t i = 0; i < symbolArguments.size(); ++i)
+    {
+        kernel->FillSymbol(loweredSymbols[i], symbolArguments[i]->GetData(), symbolArguments[i]->GetDataSize());
+    }
     compiled = true;
 
     if (m_KernelCacheCapacity > 0)
@@ -569,8 +649,10 @@ ComputeActionId HipEngine::RunKernelAsync(const KernelComputeData& data, const Q
         case ArgumentMemoryType::Local:
             dynamicLds += argument->GetDataSize();
             break;
+        case ArgumentMemoryType::Symbol:
+            break;  // in the module's variable since the kernel was loaded; no parameter slot
         default:
-            throw KttException("Symbol arguments are not supported by the HIP engine");
+            throw KttException("Unhandled argument memory type");
         }
     }
     // Second pass: parameter slots in argument order (device pointer storage is now stable).

@@ -53,6 +53,9 @@ public:
     const KernelCompilationData& GetCompilationData() const { return m_CompilationData; }
     Nanoseconds GetCompilationTime() const { return m_CompilationTime; }
     size_t GetStaticLdsSize() const { return m_CompilationData.m_LocalMemorySize; }
+    /** Copies bytes of host data into the module's global variable loweredName (a symbol
+     *  argument); the variable must hold at least that many bytes. */
+    void FillSymbol(const std::string& loweredName, const void* data, const size_t bytes) const;
 
 private:
     std::string m_Name;
@@ -70,9 +73,15 @@ public:
      *  With KTT_HIP_CACHE_DIR set, code objects are cached on disk under that directory. */
     static std::vector<char> Compile(const std::string& source, const std::string& programName, const std::string& kernelName,
         const std::vector<std::string>& options, std::string& loweredName, std::string* log = nullptr);
+    /** The same, also lowering the names of global variables (symbol arguments): loweredSymbols
+     *  receives the name of each symbols[i] in the code object. */
+    static std::vector<char> Compile(const std::string& source, const std::string& programName, const std::string& kernelName,
+        const std::vector<std::string>& options, const std::vector<std::string>& symbols, std::string& loweredName,
+        std::vector<std::string>& loweredSymbols, std::string* log = nullptr);
     /** The hipRTC compilation itself, without the cache. */
     static std::vector<char> CompileWithRtc(const std::string& source, const std::string& programName, const std::string& kernelName,
-        const std::vector<std::string>& options, std::string& loweredName, std::string* log = nullptr);
+        const std::vector<std::string>& options, std::string& loweredName, std::string* log = nullptr,
+        const std::vector<std::string>& symbols = {}, std::vector<std::string>* loweredSymbols = nullptr);
     /** Default options for the current device (or gfx950 when none is visible). */
     static std::vector<std::string> DefaultOptions(const std::string& architecture);
     /** Architecture of the device without feature suffixes, e.g. "gfx950". */

@@ -130,6 +130,22 @@ ArgumentId ArgumentRegistry::Add(const ArgumentSpec& spec, const ArgumentSource 
     {
         throw KttException("An argument named " + id + " exists already");
     }
+    if (spec.kind == ArgumentMemoryType::Symbol)
+    {
+        // Source/KernelArgument/KernelArgumentManager.cpp:122-133: symbol names are unique. On HIP the
+        // name is how the engine finds the variable in the compiled module, so it is required.
+        if (spec.symbolName.empty())
+        {
+            throw KttException("Symbol argument " + id + " needs the name of its __device__ or __constant__ variable");
+        }
+        for (const auto& [existing, argument] : m_Arguments)
+        {
+            if (argument->GetMemoryType() == ArgumentMemoryType::Symbol && argument->GetSymbolName() == spec.symbolName)
+            {
+                throw KttException("Kernel argument with symbol name " + spec.symbolName + " already exists");
+            }
+        }
+    }
 
     auto argument = std::make_unique<KernelArgument>(id, spec.elementSize, spec.dataType, spec.location, spec.access, spec.kind,
         spec.management);
@@ -145,6 +161,7 @@ ArgumentId ArgumentRegistry::Add(const ArgumentSpec& spec, const ArgumentSource 
         argument->SetUserBuffer(data, bytes);
         break;
     }
+    argument->SetSymbolName(spec.symbolName);
     m_Arguments.emplace(id, std::move(argument));
     return id;
 }

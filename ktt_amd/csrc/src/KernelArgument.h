@@ -47,6 +47,9 @@ public:
     /** Host data (copy or reference); throws for a user device buffer. */
     const void* GetData() const;
     void* GetData();
+    /** Name of the kernel's __device__ / __constant__ variable a symbol argument fills. */
+    const std::string& GetSymbolName() const { return m_SymbolName; }
+    void SetSymbolName(const std::string& name) { m_SymbolName = name; }
 
 private:
     ArgumentId m_Id;
@@ -61,6 +64,7 @@ private:
     void* m_Host;
     ComputeBuffer m_Device;
     size_t m_Bytes;
+    std::string m_SymbolName;
 };
 
 /** The vector arguments among `arguments`, in order. */
@@ -82,6 +86,7 @@ struct ArgumentSpec
     ArgumentAccessType access = ArgumentAccessType::ReadOnly;
     ArgumentMemoryType kind = ArgumentMemoryType::Vector;
     ArgumentManagementType management = ArgumentManagementType::Framework;
+    std::string symbolName;  // symbol arguments only
 };
 
 class ArgumentRegistry

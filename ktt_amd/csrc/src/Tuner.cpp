@@ -205,6 +205,23 @@ ArgumentId Tuner::RegisterArgument(const size_t elementSize, const ArgumentDataT
     });
 }
 
+ArgumentId Tuner::RegisterSymbolArgument(const size_t elementSize, const ArgumentDataType dataType, const void* data,
+    const size_t bytes, const ArgumentId& name, const std::string& symbolName)
+{
+    return Guarded(InvalidArgumentId, [&]()
+    {
+        ArgumentSpec spec;
+        spec.elementSize = elementSize;
+        spec.dataType = dataType;
+        spec.location = ArgumentMemoryLocation::Undefined;
+        spec.access = ArgumentAccessType::ReadOnly;
+        spec.kind = ArgumentMemoryType::Symbol;
+        spec.management = ArgumentManagementType::Framework;
+        spec.symbolName = symbolName;
+        return m_Tuner->GetArguments().Add(spec, ArgumentSource::Copy, const_cast<void*>(data), bytes, name);
+    });
+}
+
 ArgumentId Tuner::AddArgumentVector(ComputeBuffer buffer, const size_t bufferSize, const size_t elementSize,
     const ArgumentAccessType accessType, const ArgumentMemoryLocation memoryLocation, const ArgumentId& customId)
 {
