@@ -19,16 +19,22 @@ Beside the headline, the same line reports (each timed with HIP events around th
                plain and compensated;
   batched      1,024 independent systems (atoms log-uniform in [200, 20000], sides in
                {32, 48, 64, 96, 128}) in one graph-launched persistent kernel, against one plain
-               launch per system.
+               launch per system;
+  dynamic      192^3 grid, 131,072 atoms, 500 steps of N(0, 0.05^2 I) jitter on the device, ktt
+               TuneIteration (MCMC, 48 steps) then the best configuration; the best configuration
+               of a whole-space search (profiles/coulomb3d_dynamic_192x131072_exhaustive.json) is
+               re-timed on the same box for the converged / exhaustive ratio;
+  reference_workload  the reference example's own workload (256^3, 64 atoms U[0,20)^3,
+               q in [0, 0.5)), the only one with a published time (769.696 us, RTX 2080 Ti).
 
 The CPU baseline leg times the oracle's FP32 restatement of the reference CPU generator
 (Examples/JSON/generate-coulomb-output.c:22-31) on all host cores over a seeded sample of
-z-slices of the headline workload, and checks the GPU results (headline and precision
-workloads) against the FP64 oracle at sampled grid points.
+z-slices of the headline workload, and checks every leg's GPU results against the FP64 oracle
+at sampled grid points: max and RMS of |V - V64| / sum_i |q_i| / r_i per workload.
 
---dynamic runs the dynamic-tuning workload instead (192^3 grid, 131,072 atoms, 500 steps of
-N(0, 0.05^2 I) jitter, ktt TuneIteration steps then the best configuration) and prints its own
-line; --exhaustive adds a whole-space deterministic run for the comparison.
+--dynamic runs only the dynamic-tuning workload and prints its own line; --exhaustive adds a
+whole-space deterministic run for the comparison (it writes the exhaustive file above with
+--results-prefix).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--dynamic [--exhaustive]]
 """
@@ -46,17 +52,38 @@ import numpy as np
 
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
+# ktt logs every tuning step at Info level; the bench prints its own progress.
+os.environ.setdefault("KTT_LOG_LEVEL", "Warning")
 
 FLOP_PER_INTERACTION = 11
 PEAK_FP32_FLOPS = 157.3e12
 METRIC = "grid·atom interactions/s, plain FP32 vs compensated+batched, err ≤1e-6"
 TRAFFIC_SUMMARY = REPO / "profiles" / "coulomb3d_compensated_256x65536_traffic.json"
-PRECISION_POINTS = 4096
+EXHAUSTIVE_SUMMARY = REPO / "profiles" / "coulomb3d_dynamic_192x131072_exhaustive.json"
+SAMPLE_POINTS = 4096          # FP64-checked grid points per 3D workload
+BATCH_SAMPLE_POINTS = 64      # per system of the batch
+# The reference's only published time for this path: CoulombSum3d, 256^3 grid x 64 atoms, best of
+# the 210-configuration CUDA search on an RTX 2080 Ti
+# (Examples/CoulombSum3d/coulomb_2080_full_search_space.json:4,5434).
+PUBLISHED_US = 769.696
+PUBLISHED_DEVICE = "NVIDIA GeForce RTX 2080 Ti (KTT 2.1.0, CUDA)"
+LEGS = ("slice2d", "iterative", "precision", "batched", "reference_workload", "nbody", "dynamic")
 
 
-def cpu_baseline_leg(work, gpu_grid: np.ndarray, precision: dict | None, min_seconds: float = 10.0,
-                     max_slices: int = 24) -> tuple[dict, dict]:
-    """Times the oracle on z-slices of the workload and checks GPU results against FP64."""
+def sample(atoms, spacing, points, grid=None, values=None):
+    """GPU values at grid indices, kept for the FP64 check of the CPU-baseline leg. points are
+    (x, y, z) rows (or (x, y) for a slice); values are read from grid when not given."""
+    if values is None:
+        if points.shape[1] == 3:
+            values = grid[points[:, 2], points[:, 1], points[:, 0]]
+        else:
+            values = grid[points[:, 1], points[:, 0]]
+    return {"atoms": atoms, "spacing": spacing, "points": points, "values": np.array(values, dtype=np.float32)}
+
+
+def cpu_baseline_leg(work, samples: dict, min_seconds: float = 10.0, max_slices: int = 24) -> tuple[dict, dict]:
+    """Times the oracle on z-slices of the headline workload, and checks every leg's GPU samples
+    against FP64: samples maps a workload name to a list of sample() records."""
     from ktt_amd import workloads
     from oracle import coulomb_oracle as oracle
 
@@ -88,20 +115,31 @@ def cpu_baseline_leg(work, gpu_grid: np.ndarray, precision: dict | None, min_sec
                                 "sample": f"the whole 256^2 x 4,000 slice ({slice_work.interactions:.3e} interactions, "
                                           f"{slice_seconds:.2f} s)"}
     baseline["opencl_cpu_device"] = opencl_cpu_device()
-    points = workloads.sample_points(33, 256, work.grid)
-    start = time.perf_counter()
-    values, norms = oracle.points_f64(work.atoms, work.spacing, points)
-    fp64_seconds = time.perf_counter() - start
-    baseline["fp64_oracle"] = {"value": points.shape[0] * work.atoms.shape[0] / fp64_seconds, "unit": "interactions/s",
-                               "sample": f"{points.shape[0]} grid points of the headline workload (for information)"}
-    worst, rms = oracle.normalised_error(gpu_grid[points[:, 2], points[:, 1], points[:, 0]], values, norms)
-    errors = {"headline": {"max": worst, "rms": rms, "points": int(points.shape[0]), "against": "FP64 oracle"}}
-    if precision is not None:
-        values, norms = oracle.points_f64(precision["atoms"], precision["spacing"], precision["points"])
-        for form in ("plain", "compensated"):
-            worst, rms = oracle.normalised_error(precision["samples"][form], values, norms)
-            errors[f"precision_{form}"] = {"max": worst, "rms": rms, "points": int(precision["points"].shape[0]),
-                                           "against": "FP64 oracle"}
+
+    errors = {}
+    fp64_points, fp64_seconds = 0, 0.0
+    for name, records in samples.items():
+        worst, squares, count = 0.0, 0.0, 0
+        for record in records:
+            start = time.perf_counter()
+            if record["points"].shape[1] == 3:
+                values, norms = oracle.points_f64(record["atoms"], record["spacing"], record["points"])
+            else:
+                values, norms = oracle.points2d_f64(record["atoms"], record["spacing"], record["points"])
+            if name == "headline":
+                fp64_points += record["points"].shape[0]
+                fp64_seconds += time.perf_counter() - start
+            error = np.abs(record["values"].astype(np.float64) - values) / norms
+            worst = max(worst, float(error.max()))
+            squares += float(np.sum(error ** 2))
+            count += error.size
+        errors[name] = {"max": worst, "rms": float(np.sqrt(squares / count)), "points": count,
+                        "systems": len(records), "against": "FP64 oracle, normalised by sum_i |q_i| / r_i"}
+        if len(records) == 1:
+            del errors[name]["systems"]
+    if fp64_seconds > 0:
+        baseline["fp64_oracle"] = {"value": fp64_points * work.atoms.shape[0] / fp64_seconds, "unit": "interactions/s",
+                                   "sample": f"{fp64_points} grid points of the headline workload (for information)"}
     return baseline, errors
 
 
@@ -136,7 +174,7 @@ def timed_loop(capi, launch, steps: int, warmup: int) -> tuple[float, float]:
     return start.elapsed_ms(end) / steps, 1e3 * wall / steps
 
 
-def headline_leg(capi, workloads, args):
+def headline_leg(capi, workloads, args, samples: dict):
     work = workloads.volume3d()
     n_atoms = work.atoms.shape[0]
     atoms = capi.DeviceBuffer.from_array(work.atoms)
@@ -147,9 +185,9 @@ def headline_leg(capi, workloads, args):
     energy.zero()
     calibration.copy_from(energy)
     capi.synchronize()
+    points = workloads.sample_points(33, SAMPLE_POINTS, work.grid)
 
     results = {}
-    grid_values = None
     for name, form in (("plain", capi.PLAIN), ("compensated", capi.COMPENSATED)):
         energy.zero()
 
@@ -158,29 +196,34 @@ def headline_leg(capi, workloads, args):
 
         per_launch, wall = timed_loop(capi, launch, args.steps, args.warmup)
         results[name] = {"ms_per_step": per_launch, "wall_ms_per_step": wall, "value": work.interactions / (per_launch * 1e-3)}
-        if name == "compensated":
-            # One clean pass for the error check.
-            energy.zero()
-            launch()
-            capi.synchronize()
-            grid_values = energy.download((work.grid, work.grid, work.grid))
+        # One clean pass for the error check.
+        energy.zero()
+        launch()
+        capi.synchronize()
+        grid = energy.download((work.grid, work.grid, work.grid))
+        samples["headline" if name == "compensated" else "headline_plain"] = [sample(work.atoms, work.spacing, points, grid)]
 
     for buffer in (atoms, calibration, energy):
         buffer.close()
-    return work, results, grid_values
+    return work, results
 
 
-def iterative_leg(capi, work) -> dict:
+def iterative_leg(capi, workloads, work, samples: dict) -> dict:
     """CoulombSum3dIterative on the headline inputs: the reference's host dz^2 path (one upload
     and launch per slice) and the fused device path (one launch per slice)."""
     n_atoms = work.atoms.shape[0]
     atoms = capi.DeviceBuffer.from_array(work.atoms)
     energy = capi.DeviceBuffer(4 * work.grid ** 3)
+    points = workloads.sample_points(36, SAMPLE_POINTS, work.grid)
     energy.zero()
     capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
     energy.zero()
     kernel_ms, wall_ms = capi.coulomb3d_iterative(work.atoms, work.spacing, work.grid, energy)
-    slices_ms, _ = timed_loop(capi, lambda: capi.coulomb3d_slices(atoms, n_atoms, work.spacing, work.grid, energy, timed=False), 1, 1)
+    samples["iterative_host"] = [sample(work.atoms, work.spacing, points, energy.download((work.grid,) * 3))]
+    capi.coulomb3d_slices(atoms, n_atoms, work.spacing, work.grid, energy, timed=False)  # compiles the slice kernel
+    energy.zero()
+    slices_ms, _ = timed_loop(capi, lambda: capi.coulomb3d_slices(atoms, n_atoms, work.spacing, work.grid, energy, timed=False), 1, 0)
+    samples["iterative_device"] = [sample(work.atoms, work.spacing, points, energy.download((work.grid,) * 3))]
     atoms.close()
     energy.close()
     return {
@@ -190,40 +233,46 @@ def iterative_leg(capi, work) -> dict:
     }
 
 
-def slice2d_leg(capi, workloads, args) -> dict:
+def slice2d_leg(capi, workloads, args, samples: dict) -> dict:
     work = workloads.slice2d()
     atoms = capi.DeviceBuffer.from_array(work.slice_atoms)
     energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
     energy.zero()
     per_launch, wall = timed_loop(capi, lambda: capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x,
-                                                               work.grid_y, energy, timed=False), max(args.steps, 20), args.warmup)
+                                                               work.grid_y, energy, timed=False), max(args.steps, 20),
+                                  max(args.warmup, 1))
+    energy.zero()
+    capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy)
+    y, x = np.meshgrid(np.arange(work.grid_y), np.arange(work.grid_x), indexing="ij")
+    points = np.stack([x.ravel(), y.ravel()], axis=1).astype(np.int32)
+    samples["slice2d"] = [sample(work.slice_atoms, work.spacing, points, energy.download((work.grid_y, work.grid_x)))]
     atoms.close()
     energy.close()
     return {"ms": per_launch, "wall_ms": wall, "value": work.interactions / (per_launch * 1e-3),
-            "interactions": work.interactions}
+            "interactions": work.interactions, "frac_of_fp32_peak": work.interactions / (per_launch * 1e-3)
+            * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS}
 
 
-def precision_leg(capi, workloads) -> tuple[dict, dict]:
+def precision_leg(capi, workloads, samples: dict) -> dict:
     work = workloads.clustered()
     n_atoms = work.atoms.shape[0]
     atoms = capi.DeviceBuffer.from_array(work.atoms)
     energy = capi.DeviceBuffer(4 * work.grid ** 3)
-    points = workloads.sample_points(3, PRECISION_POINTS, work.grid)
-    timing, samples = {}, {}
+    points = workloads.sample_points(3, SAMPLE_POINTS, work.grid)
+    timing = {}
     for name, form in (("plain", capi.PLAIN), ("compensated", capi.COMPENSATED)):
         energy.zero()
         ms = capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form)
-        grid = energy.download((work.grid, work.grid, work.grid))
-        samples[name] = grid[points[:, 2], points[:, 1], points[:, 0]].copy()
+        samples[f"precision_{name}"] = [sample(work.atoms, work.spacing, points, energy.download((work.grid,) * 3))]
         timing[name] = {"ms": ms, "value": work.interactions / (ms * 1e-3)}
     atoms.close()
     energy.close()
     timing["atoms"] = n_atoms
     timing["interactions"] = work.interactions
-    return timing, {"atoms": work.atoms, "spacing": work.spacing, "points": points, "samples": samples}
+    return timing
 
 
-def batched_leg(capi, workloads, config=None, repeats: int = 3) -> dict:
+def batched_leg(capi, workloads, samples: dict, config=None, repeats: int = 3) -> dict:
     systems = workloads.batch()
     batch = capi.Batch(systems, config=config)
     info = batch.info()
@@ -233,6 +282,12 @@ def batched_leg(capi, workloads, config=None, repeats: int = 3) -> dict:
     for _ in range(repeats):
         batch.zero()
         graph.append(batch.run(use_graph=True))
+    # The last graph run's grids, at BATCH_SAMPLE_POINTS points of every system.
+    records = []
+    for k, (system, grid) in enumerate(zip(systems, batch.grids())):
+        points = workloads.sample_points(4000 + k, BATCH_SAMPLE_POINTS, system.grid)
+        records.append(sample(system.atoms, system.spacing, points, grid))
+    samples["batched"] = records
     batch.zero()
     batch.run_per_system(form=capi.PLAIN)
     per_system = []
@@ -249,11 +304,100 @@ def batched_leg(capi, workloads, config=None, repeats: int = 3) -> dict:
         "shapes": info["shapes"],
         "graph_ms": graph_ms,
         "value": info["interactions"] / (graph_ms * 1e-3),
+        "frac_of_fp32_peak": info["interactions"] / (graph_ms * 1e-3) * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
         "per_system_plain_ms": per_system_ms,
         "per_system_plain_value": info["interactions"] / (per_system_ms * 1e-3),
         "speedup_vs_per_system_plain": per_system_ms / graph_ms,
         "form": "compensated, one persistent launch from a captured HIP graph",
     }
+
+
+def exhaustive_summary() -> dict | None:
+    if not EXHAUSTIVE_SUMMARY.exists():
+        return None
+    return json.loads(EXHAUSTIVE_SUMMARY.read_text())
+
+
+def dynamic_leg(capi, workloads, samples: dict, steps: int, budget: int, retime_steps: int = 5) -> dict:
+    """The dynamic workload: ktt TuneIteration steps (MCMC) then the best configuration, with the
+    atoms jittered on the device every step. The converged rate is the median production step;
+    the committed whole-space search's best configuration is re-timed here on the last step's
+    atoms for the converged / exhaustive ratio."""
+    work = workloads.dynamic_initial()
+    run = capi.coulomb3d_dynamic(work.atoms, work.spacing, work.grid, steps, workloads.DYNAMIC_SIGMA, workloads.SEED_DYNAMIC,
+                                 searcher=capi.SEARCHER_MCMC, tuning_budget=budget, want_final=True)
+    production = run["kernel_ms"][~run["tuned"]]
+    converged_ms = float(np.median(production))
+    points = workloads.sample_points(79, SAMPLE_POINTS, work.grid)
+    samples["dynamic"] = [sample(run["atoms"], work.spacing, points, run["grid"])]
+    value = work.interactions / (converged_ms * 1e-3)
+    result = {
+        "workload": f"{work.grid}^3 grid, h=0.5 A, {work.atoms.shape[0]} atoms U in the box, q U[-1,1]; {steps} steps of "
+                    f"N(0, {workloads.DYNAMIC_SIGMA}^2 I) jitter on the device; compensated FP32",
+        "steps": steps,
+        "tuning_steps": int(run["tuned"].sum()),
+        "space": run["space"],
+        "searcher": "MCMC (ktt TuneIteration)",
+        "best_config": run["best_config"],
+        "converged_ms": converged_ms,
+        "value": value,
+        "frac_of_fp32_peak": value * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
+        "tuning_wall_s": float(run["wall_ms"][run["tuned"]].sum() / 1e3),
+        "total_wall_s": float(run["wall_ms"].sum() / 1e3),
+        "exhaustive": None,
+    }
+    summary = exhaustive_summary()
+    if summary is not None:
+        atoms = capi.DeviceBuffer.from_array(run["atoms"])
+        energy = capi.DeviceBuffer(4 * work.grid ** 3)
+        energy.zero()
+        timings = {}
+        # The dynamic launcher splits every tile over ATOM_BLOCKS (no tail split): the same here.
+        for name, config in (("exhaustive_best", summary["best_config"]), ("converged", run["best_config"])):
+            launch = lambda config=config: capi.coulomb3d(atoms, work.atoms.shape[0], work.spacing, work.grid, energy,
+                                                          form=capi.COMPENSATED, config=config + ",TAIL_SPLIT=0", timed=False)
+            timings[name], _ = timed_loop(capi, launch, retime_steps, 1)
+        atoms.close()
+        energy.close()
+        result["exhaustive"] = {
+            "best_config": summary["best_config"],
+            "searched": summary["tested"],
+            "source": str(EXHAUSTIVE_SUMMARY.relative_to(REPO)),
+            "retimed_ms": timings["exhaustive_best"],
+            "converged_retimed_ms": timings["converged"],
+            "converged_over_exhaustive": timings["converged"] / timings["exhaustive_best"],
+        }
+    return result
+
+
+def reference_workload_leg(capi, workloads, args, samples: dict) -> dict:
+    """The reference example's own workload (256^3 x 64 atoms in a 20 A corner, q in [0, 0.5)),
+    beside the reference's only published time for it."""
+    work = workloads.reference_example()
+    n_atoms = work.atoms.shape[0]
+    atoms = capi.DeviceBuffer.from_array(work.atoms)
+    energy = capi.DeviceBuffer(4 * work.grid ** 3)
+    points = workloads.sample_points(37, SAMPLE_POINTS, work.grid)
+    result = {"workload": "Examples/CoulombSum3d/CoulombSum3d.cpp:67-98: 256^3 grid, h=0.5 A, 64 atoms U[0,20)^3, "
+                          "q U[0,20)/40, seeded",
+              "interactions": work.interactions,
+              "published": {"us": PUBLISHED_US, "value": work.interactions / (PUBLISHED_US * 1e-6), "device": PUBLISHED_DEVICE,
+                            "source": "Examples/CoulombSum3d/coulomb_2080_full_search_space.json:4,5434"}}
+    for name, form in (("plain", capi.PLAIN), ("compensated", capi.COMPENSATED)):
+        energy.zero()
+        launch = lambda form=form: capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form, timed=False)
+        per_launch, _ = timed_loop(capi, launch, max(args.steps, 20), max(args.warmup, 1))
+        energy.zero()
+        launch()
+        capi.synchronize()
+        samples[f"reference_workload_{name}"] = [sample(work.atoms, work.spacing, points, energy.download((work.grid,) * 3))]
+        value = work.interactions / (per_launch * 1e-3)
+        result[name] = {"us": per_launch * 1e3, "value": value, "frac_of_fp32_peak": value * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
+                        "vs_published": value / result["published"]["value"]}
+    result["note"] = "vs_published compares one MI355X with the RTX 2080 Ti figure: different hardware, same workload"
+    atoms.close()
+    energy.close()
+    return result
 
 
 NBODY_BODIES = 128 * 1024  # Examples/Nbody/Nbody.cpp:60
@@ -356,6 +500,14 @@ def dynamic_main(args) -> int:
         line["exhaustive"] = {"best_config": exhaustive["best_config"], "ms": best_ms,
                               "value": work.interactions / (best_ms * 1e-3), "tested": exhaustive["tested"],
                               "converged_over_exhaustive": best_ms / converged_ms}
+        if args.exhaustive_out:
+            summary = {"workload": f"dynamic: {work.grid}^3 grid, h=0.5 A, {work.atoms.shape[0]} atoms U in box, q U[-1,1], "
+                                   f"N(0, {workloads.DYNAMIC_SIGMA}^2 I) jitter per step on the device; compensated FP32; "
+                                   f"Deterministic TuneIteration over all {exhaustive['space']} configurations, then the best",
+                       "best_config": exhaustive["best_config"], "best_ms": best_ms,
+                       "tested": exhaustive["tested"], "space": exhaustive["space"], "device": capi.device_info()["name"],
+                       "command": "python bench.py --dynamic --exhaustive --exhaustive-out <file>"}
+            Path(args.exhaustive_out).write_text(json.dumps(summary, indent=1) + "\n")
     print(json.dumps(line))
     return 0
 
@@ -369,11 +521,13 @@ def main() -> int:
     parser.add_argument("--batch-config", default=None, help="per-bucket shapes BUCKETb_WGX=...,... (default: built-in)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-extras", action="store_true", help="headline and CPU baseline only")
+    parser.add_argument("--legs", default="all", help="comma-separated legs beside the headline: " + ",".join(LEGS))
     parser.add_argument("--dynamic", action="store_true", help="run the dynamic-tuning workload instead")
     parser.add_argument("--exhaustive", action="store_true", help="with --dynamic: add a whole-space deterministic run")
-    parser.add_argument("--dynamic-steps", type=int, default=500)
+    parser.add_argument("--dynamic-steps", type=int, default=500, help="steps of the dynamic workload (0: skip its leg)")
     parser.add_argument("--dynamic-budget", type=int, default=48)
     parser.add_argument("--results-prefix", default=None, help="with --dynamic: KTT JSON results files prefix")
+    parser.add_argument("--exhaustive-out", default=None, help="with --exhaustive: write the whole-space search's best here")
     args = parser.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -388,19 +542,55 @@ def main() -> int:
 
     capi.library()
     info = capi.device_info()
-    work, results, grid_values = headline_leg(capi, workloads, args)
+    samples = {}
+    work, results = headline_leg(capi, workloads, args, samples)
     progress(f"headline: compensated {results['compensated']['ms_per_step']:.2f} ms, plain {results['plain']['ms_per_step']:.2f} ms")
-    extras, precision_samples = {}, None
-    if not args.no_extras:
-        extras["slice2d"] = slice2d_leg(capi, workloads, args)
-        extras["iterative"] = iterative_leg(capi, work)
-        progress("slice2d and iterative done")
-        extras["precision"], precision_samples = precision_leg(capi, workloads)
+    extras = {}
+    rates = capi.issue_rates()
+    # The mix probe issues 8 v_rsq_f32 and 16 v_fma_f32 per round with no dependence between them:
+    # if the two kinds shared the issue port without overlap, its rate would be 24 / (8/rsq + 16/fma).
+    serial_mix = 24.0 / (8.0 / rates["rsq"] + 16.0 / rates["fma"])
+    # One interaction needs 2 FMA-class instructions and one v_rsq_f32 (per lane).
+    ceiling = 1.0 / (2.0 / rates["fma"] + 1.0 / rates["rsq"])
+    extras["issue_rates"] = {
+        "unit": "lane operations/s", **rates,
+        "rsq_cost_in_fma_slots": rates["fma"] / rates["rsq"],
+        "mix_if_serial": serial_mix,
+        "mix_over_serial": rates["mix"] / serial_mix,
+        "coulomb_ceiling_interactions_per_s": ceiling,
+        "coulomb_ceiling_frac_of_fp32_peak": ceiling * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
+        "headline_over_ceiling": results["compensated"]["value"] / ceiling,
+    }
+    progress(f"issue rates: fma {rates['fma']:.3e}, rsq {rates['rsq']:.3e}, mix {rates['mix']:.3e} lane-ops/s")
+    legs = set(LEGS) if args.legs == "all" else {leg for leg in args.legs.split(",") if leg}
+    if args.no_extras:
+        legs = set()
+    unknown = legs - set(LEGS)
+    if unknown:
+        raise SystemExit(f"unknown legs {sorted(unknown)}; choose from {', '.join(LEGS)}")
+    if "slice2d" in legs:
+        extras["slice2d"] = slice2d_leg(capi, workloads, args, samples)
+        progress(f"slice2d {extras['slice2d']['ms'] * 1e3:.1f} us")
+    if "iterative" in legs:
+        extras["iterative"] = iterative_leg(capi, workloads, work, samples)
+        progress(f"iterative host {extras['iterative']['host_dz2']['kernel_ms']:.1f} ms, "
+                 f"device {extras['iterative']['device_slices']['ms']:.1f} ms")
+    if "precision" in legs:
+        extras["precision"] = precision_leg(capi, workloads, samples)
         progress("precision workload done")
-        extras["batched"] = batched_leg(capi, workloads, config=args.batch_config)
+    if "batched" in legs:
+        extras["batched"] = batched_leg(capi, workloads, samples, config=args.batch_config)
         progress(f"batched: graph {extras['batched']['graph_ms']:.1f} ms, per system {extras['batched']['per_system_plain_ms']:.1f} ms")
-        extras["nbody"] = nbody_leg(capi, max(args.steps, 5), args.warmup)
+    if "reference_workload" in legs:
+        extras["reference_workload"] = reference_workload_leg(capi, workloads, args, samples)
+        progress(f"reference workload: plain {extras['reference_workload']['plain']['us']:.1f} us")
+    if "nbody" in legs:
+        extras["nbody"] = nbody_leg(capi, max(args.steps, 5), max(args.warmup, 1))
         progress(f"nbody: {extras['nbody']['ms']:.3f} ms, {extras['nbody']['tflops']:.1f} TFLOP/s")
+    if "dynamic" in legs and args.dynamic_steps > 0:
+        progress(f"dynamic: {args.dynamic_steps} steps, {args.dynamic_budget} tuning")
+        extras["dynamic"] = dynamic_leg(capi, workloads, samples, args.dynamic_steps, args.dynamic_budget)
+        progress(f"dynamic: converged {extras['dynamic']['converged_ms']:.2f} ms")
 
     headline = results["compensated"]
     achieved = headline["value"] * FLOP_PER_INTERACTION / 1e12
@@ -432,6 +622,8 @@ def main() -> int:
         "compensated": results["compensated"],
         "roofline": {
             "bound": "mfma",
+            "bound_note": "compute-bound: 'mfma' is the compute side of the record's two values; the pipe doing the "
+                          "work is the FP32 VALU (the kernel issues no MFMA instruction)",
             "pipe": "FP32 VALU (v_rsq_f32 + FMA); no MFMA, the sum is not a contraction",
             "achieved": achieved,
             "peak": PEAK_FP32_FLOPS / 1e12,
@@ -445,7 +637,7 @@ def main() -> int:
     line.update(extras)
     if not args.no_cpu_baseline:
         progress("CPU baseline and FP64 checks")
-        baseline, errors = cpu_baseline_leg(work, grid_values, precision_samples)
+        baseline, errors = cpu_baseline_leg(work, samples)
         line["cpu_baseline"] = baseline
         line["error"] = errors
     else:

@@ -65,6 +65,13 @@ int ktt_hip_zero(void* device_pointer, size_t bytes, void* stream);
 /* Device-to-device copy with a plain 16-byte load/store kernel (16-byte aligned pointers, bytes a
  * multiple of 16); its profiler counters calibrate the HBM traffic figures of the benchmark. */
 int ktt_hip_copy(void* device_destination, const void* device_source, size_t bytes, void* stream);
+/* Issue rates of the current device's vector ALU, in lane operations per second (instructions
+ * x 64 lanes), from probe kernels of 8 independent chains per thread at 8 waves per SIMD:
+ * v_fma_f32 alone, v_rsq_f32 alone, and the Coulomb inner loop's mix of 2 FMA + 1 v_rsq_f32.
+ * Replaces nothing in the reference: it calibrates the transcendental weight of the CDNA
+ * bottleneck analysis (the CUDA one reads the XU pipe's utilisation from CUPTI,
+ * Source/Api/Searcher/ProfileBasedSearcher.h:123-340). Each output optional. */
+int ktt_hip_issue_rates(int iterations, double* fma_rate, double* rsq_rate, double* mix_rate);
 int ktt_hip_synchronize(void);
 int ktt_hip_stream_create(void** stream);
 int ktt_hip_stream_destroy(void* stream);
@@ -90,6 +97,15 @@ int ktt_hip_event_elapsed_ms(void* start, void* end, float* milliseconds);
  * the last, partial round of workgroups over atom blocks; 0 does not). */
 int ktt_coulomb3d(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                   int form, const char* config, void* stream, float* kernel_ms);
+
+/* The launch ktt_coulomb3d makes for these arguments, without launching: the grid of workgroups
+ * (z = atom blocks; with the compensated form's tail split, the blocks the split tiles use), the
+ * workgroup shape, the first split tile (tiles are numbered y * grid[0] + x; equal to the tile
+ * count when nothing is split) and the device scratch of the split in bytes. Compiles the kernel
+ * (the split depends on its occupancy), so it needs the device. Each output optional. Replaces
+ * nothing in the reference: the occupancy-driven split has no counterpart there. */
+int ktt_coulomb3d_launch_plan(int n_atoms, int grid_size, int form, const char* config, unsigned int grid[3],
+                              unsigned int block[3], int* split_from, size_t* scratch_bytes);
 
 /* ktt_coulomb3d in the compensated form with the default configuration, allocating nothing.
  * No counterpart in the reference: it is for callers that cannot allocate device memory (code

@@ -58,6 +58,9 @@ KERNEL_CHECKS = [
     ("coulomb_util.hip", "kttCopy16", None),
     ("coulomb_util.hip", "kttZeroFill4", None),
     ("coulomb_util.hip", "kttJitterAtoms", None),
+    ("coulomb_util.hip", "kttRateFma", None),
+    ("coulomb_util.hip", "kttRateRsq", None),
+    ("coulomb_util.hip", "kttRateMix", None),
     ("CoulombSum3d.hip", "directCoulombSumCompensated",
      "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=8,POINTS_X=6,ROWS_PER_THREAD=4,ATOM_TILE=128"),
     ("CoulombSum3d.hip", "directCoulombSum",

@@ -39,6 +39,7 @@ SIGNATURES = {
     "ktt_hip_memcpy_dtoh": (_c.c_int, [_vp, _vp, _c.c_size_t]),
     "ktt_hip_zero": (_c.c_int, [_vp, _c.c_size_t, _vp]),
     "ktt_hip_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
+    "ktt_hip_issue_rates": (_c.c_int, [_c.c_int, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
     "ktt_hip_synchronize": (_c.c_int, []),
     "ktt_hip_stream_create": (_c.c_int, [_c.POINTER(_vp)]),
     "ktt_hip_stream_destroy": (_c.c_int, [_vp]),
@@ -48,6 +49,8 @@ SIGNATURES = {
     "ktt_hip_event_elapsed_ms": (_c.c_int, [_vp, _vp, _fp]),
     "ktt_coulomb3d": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _c.c_int, _c.c_char_p, _vp, _fp]),
     "ktt_coulomb3d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _vp, _fp]),
+    "ktt_coulomb3d_launch_plan": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_char_p, _uint3, _uint3, _c.POINTER(_c.c_int),
+                                             _c.POINTER(_c.c_size_t)]),
     "ktt_coulomb3d_compensated_noalloc": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _vp, _vp]),
     "ktt_coulomb2d": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _c.c_char_p, _vp, _fp]),
     "ktt_coulomb2d_reference": (_c.c_int, [_vp, _c.c_int, _c.c_float, _c.c_int, _c.c_int, _vp, _vp, _fp]),
@@ -198,6 +201,14 @@ class DeviceBuffer:
             pass
 
 
+def issue_rates(iterations: int = 2048) -> dict:
+    """Vector-ALU issue rates of the device (lane operations per second): FMA alone, v_rsq_f32
+    alone, and the Coulomb mix of 2 FMA + 1 v_rsq_f32."""
+    rates = [ctypes.c_double(0.0) for _ in range(3)]
+    _check(library().ktt_hip_issue_rates(iterations, *(ctypes.byref(rate) for rate in rates)), "ktt_hip_issue_rates")
+    return {"fma": rates[0].value, "rsq": rates[1].value, "mix": rates[2].value}
+
+
 def synchronize() -> None:
     _check(library().ktt_hip_synchronize(), "ktt_hip_synchronize")
 
@@ -291,6 +302,18 @@ def coulomb3d_iterative_reference(atoms: DeviceBuffer, n_atoms: int, spacing: fl
     _check(library().ktt_coulomb3d_iterative_reference(atoms.ptr, n_atoms, spacing, grid, energy.ptr, stream,
                                                        ctypes.byref(ms) if timed else None), "ktt_coulomb3d_iterative_reference")
     return ms.value if timed else None
+
+
+def coulomb3d_launch_plan(n_atoms: int, grid: int, form: int = PLAIN, config=None) -> dict:
+    """The launch ktt_coulomb3d makes: workgroup grid (z = atom blocks), workgroup shape, first split
+    tile and the split's scratch bytes."""
+    g, b = _uint3(), _uint3()
+    split_from = ctypes.c_int(0)
+    scratch = ctypes.c_size_t(0)
+    _check(library().ktt_coulomb3d_launch_plan(n_atoms, grid, form, _config(config), g, b, ctypes.byref(split_from),
+                                               ctypes.byref(scratch)), "ktt_coulomb3d_launch_plan")
+    return {"grid": tuple(g), "block": tuple(b), "split_from": split_from.value, "scratch_bytes": scratch.value,
+            "tiles": g[0] * g[1]}
 
 
 def launch_shape(kind: int, grid: int, config=None) -> tuple[tuple[int, int, int], tuple[int, int, int]]:

@@ -423,6 +423,22 @@ static void CheckGrid(const float* grid, const long points)
     }
 }
 
+// A default atom tile larger than the atom count only stages zero-charge padding (64 atoms in a
+// 256-atom tile do a quarter of the tile's work): halve it while it still holds every atom and
+// stays a multiple of the bracket stage's atom stride (threads / rows per workgroup), down to 16.
+static void ShrinkTileToAtoms(CoulombConfig& config, const int atomCount)
+{
+    const long threads = config.Get("WORK_GROUP_SIZE_X", 64) * config.Get("WORK_GROUP_SIZE_Y", 4);
+    const long rows = config.Get("WORK_GROUP_SIZE_Y", 4) * config.Get("ROWS_PER_THREAD", 16);
+    const long stride = (threads > rows && rows > 0) ? threads / rows : 1;
+    long tile = config.Get("ATOM_TILE", 64);
+    while (tile % 2 == 0 && tile / 2 >= 16 && tile / 2 >= atomCount && (tile / 2) % stride == 0)
+    {
+        tile /= 2;
+    }
+    config.Set("ATOM_TILE", std::to_string(tile));
+}
+
 static long EffectiveAtomBlocks(const CoulombConfig& config, const int atomCount)
 {
     const long tile = std::max<long>(1, config.Get("ATOM_TILE", 64));
@@ -632,6 +648,50 @@ int ktt_hip_copy(void* destination, const void* source, size_t bytes, void* stre
     KTT_C_CATCH
 }
 
+int ktt_hip_issue_rates(int iterations, double* fmaRate, double* rsqRate, double* mixRate)
+{
+    KTT_C_TRY
+    if (iterations <= 0)
+    {
+        return SetLastError("iterations must be positive");
+    }
+    auto& runtime = CoulombRuntime::Get();
+    int device = 0;
+    CheckHipError(hipGetDevice(&device), "hipGetDevice");
+    int units = 0;
+    CheckHipError(hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+    // 8 workgroups of 256 threads per CU: 8 waves per SIMD, enough to hide every chain's latency.
+    const unsigned int groups = static_cast<unsigned int>(std::max(units, 1)) * 8u;
+    const size_t threads = static_cast<size_t>(groups) * 256;
+    float* out = static_cast<float*>(runtime.GetScratch("issue_rates.out", threads * sizeof(float)));
+    const char* names[3] = {"kttRateFma", "kttRateRsq", "kttRateMix"};
+    const double perRound[3] = {8.0, 8.0, 24.0};  // VALU instructions per round and thread
+    double* rates[3] = {fmaRate, rsqRate, mixRate};
+    for (int k = 0; k < 3; ++k)
+    {
+        auto kernel = runtime.GetKernel("coulomb_util.hip", names[k], {});
+        void* arguments[] = {&out, &iterations};
+        const unsigned int grid[3] = {groups, 1, 1};
+        const unsigned int block[3] = {256, 1, 1};
+        float best = 0.0f;
+        for (int repeat = 0; repeat < 4; ++repeat)  // the first launch warms up; the fastest of the rest counts
+        {
+            float milliseconds = 0.0f;
+            CoulombRuntime::Launch(*kernel, grid, block, arguments, nullptr, &milliseconds);
+            if (repeat > 0 && (best == 0.0f || milliseconds < best))
+            {
+                best = milliseconds;
+            }
+        }
+        if (rates[k] != nullptr)
+        {
+            *rates[k] = best > 0.0f ? static_cast<double>(threads) * iterations * perRound[k] / (best * 1.0e-3) : 0.0;
+        }
+    }
+    return 0;
+    KTT_C_CATCH
+}
+
 int ktt_hip_synchronize(void)
 {
     KTT_C_TRY
@@ -711,6 +771,85 @@ int ktt_coulomb_launch_shape(int kind, int gridSize, const char* config, unsigne
     KTT_C_CATCH
 }
 
+} // extern "C"
+
+namespace ktt
+{
+
+// Launch plan of one ktt_coulomb3d call: configuration, kernel, workgroup grid (z = atom blocks),
+// the first split tile and the scratch the split needs.
+struct Coulomb3dPlan
+{
+    CoulombConfig config;
+    CoulombShape shape;
+    std::shared_ptr<HipKernel> kernel;
+    unsigned int grid[3];
+    int splitFrom = 0;
+    size_t partialBytes = 0;
+    size_t counterCount = 0;
+};
+
+static Coulomb3dPlan PlanCoulomb3d(const int atomCount, const int gridSize, const int form, const char* config)
+{
+    if (gridSize <= 0 || static_cast<long>(gridSize) * gridSize > 0x7fffffffL)
+    {
+        throw KttException("grid size must be in [1, 46340]");
+    }
+    if (form != KTT_COULOMB_PLAIN && form != KTT_COULOMB_COMPENSATED)
+    {
+        throw KttException("unknown form " + std::to_string(form));
+    }
+    Coulomb3dPlan plan;
+    plan.config = CoulombConfig::Parse(config, Defaults3d(form, gridSize));
+    if (!CoulombConfig::Parse(config, CoulombConfig()).Has("ATOM_TILE"))
+    {
+        ShrinkTileToAtoms(plan.config, atomCount);
+    }
+    const long atomBlocks = EffectiveAtomBlocks(plan.config, atomCount);
+    plan.shape = CoulombRuntime::RowShape(plan.config, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks);
+    const std::vector<std::string> defines = plan.config.Defines(kLaunchOnly);
+    auto& runtime = CoulombRuntime::Get();
+    plan.grid[0] = plan.shape.grid[0];
+    plan.grid[1] = plan.shape.grid[1];
+    plan.grid[2] = plan.shape.grid[2];
+    if (form == KTT_COULOMB_PLAIN)
+    {
+        plan.kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSum", defines);
+        return plan;
+    }
+
+    plan.kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSumCompensated", defines);
+    const size_t tiles = static_cast<size_t>(plan.shape.grid[0]) * plan.shape.grid[1];
+    // ATOM_BLOCKS > 1 splits every tile.  Otherwise, with TAIL_SPLIT, only the tiles of the last
+    // round of workgroups are split: with S resident workgroups per launch (CUs x workgroups per
+    // CU at this kernel's occupancy), tiles % S tiles are left for that round, and splitting each
+    // over about S / (tiles % S) blocks fills it.
+    long blocks = atomBlocks;
+    if (atomBlocks == 1 && plan.config.Get("TAIL_SPLIT", 0) != 0)
+    {
+        const size_t slots = CoulombRuntime::ResidentWorkgroups(*plan.kernel, plan.shape.threads);
+        const size_t tail = slots > 0 ? tiles % slots : 0;
+        const long tile = plan.config.Get("ATOM_TILE", 64);
+        const long atomTiles = std::max<long>(1, (atomCount + tile - 1) / tile);
+        const long fill = tail > 0 ? static_cast<long>(slots / tail) : 1;
+        blocks = std::max<long>(1, std::min<long>({fill, atomTiles, kMaxTailBlocks}));
+        plan.splitFrom = blocks > 1 ? static_cast<int>(tiles - tail) : static_cast<int>(tiles);
+    }
+    plan.grid[2] = static_cast<unsigned int>(blocks);
+    if (blocks > 1)
+    {
+        const size_t splitTiles = tiles - static_cast<size_t>(plan.splitFrom);
+        plan.partialBytes = static_cast<size_t>(blocks) * splitTiles * static_cast<size_t>(plan.shape.threads)
+            * static_cast<size_t>(plan.shape.pointsPerThread) * 8;
+        plan.counterCount = splitTiles;  // one arrival counter per split tile
+    }
+    return plan;
+}
+
+} // namespace ktt
+
+extern "C" {
+
 int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize, float* energyGrid, int form, const char* config,
     void* stream, float* kernelMs)
 {
@@ -725,53 +864,53 @@ int ktt_coulomb3d(const float* atoms, int atomCount, float spacing, int gridSize
     {
         return SetLastError("unknown form " + std::to_string(form));
     }
-    auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, Defaults3d(form, gridSize));
-    const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
-    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks);
-    const std::vector<std::string> defines = parsed.Defines(kLaunchOnly);
+    Coulomb3dPlan plan = PlanCoulomb3d(atomCount, gridSize, form, config);
     const float* unused = nullptr;
     hipStream_t hipStream = static_cast<hipStream_t>(stream);
-
     if (form == KTT_COULOMB_PLAIN)
     {
-        auto kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSum", defines);
         void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid};
-        CoulombRuntime::Launch(*kernel, shape.grid, shape.block, arguments, hipStream, kernelMs);
+        CoulombRuntime::Launch(*plan.kernel, plan.grid, plan.shape.block, arguments, hipStream, kernelMs);
         return 0;
     }
-
-    auto kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSumCompensated", defines);
-    const size_t tiles = static_cast<size_t>(shape.grid[0]) * shape.grid[1];
-    // ATOM_BLOCKS > 1 splits every tile.  Otherwise, with TAIL_SPLIT, only the tiles of the last
-    // round of workgroups are split: with S resident workgroups per launch (CUs x workgroups per
-    // CU at this kernel's occupancy), tiles % S tiles are left for that round, and splitting each
-    // over about S / (tiles % S) blocks fills it.
-    int splitFrom = 0;
-    long blocks = atomBlocks;
-    if (atomBlocks == 1 && parsed.Get("TAIL_SPLIT", 0) != 0)
-    {
-        const size_t slots = CoulombRuntime::ResidentWorkgroups(*kernel, shape.threads);
-        const size_t tail = slots > 0 ? tiles % slots : 0;
-        const long atomTiles = std::max<long>(1, (atomCount + parsed.Get("ATOM_TILE", 64) - 1) / parsed.Get("ATOM_TILE", 64));
-        const long fill = tail > 0 ? static_cast<long>(slots / tail) : 1;
-        blocks = std::max<long>(1, std::min<long>({fill, atomTiles, kMaxTailBlocks}));
-        splitFrom = blocks > 1 ? static_cast<int>(tiles - tail) : static_cast<int>(tiles);
-    }
-    unsigned int grid[3] = {shape.grid[0], shape.grid[1], static_cast<unsigned int>(blocks)};
-    void* partials = nullptr;
-    if (blocks > 1)
-    {
-        const size_t bytes = static_cast<size_t>(blocks) * (tiles - static_cast<size_t>(splitFrom)) * static_cast<size_t>(shape.threads)
-            * static_cast<size_t>(shape.pointsPerThread) * 8;
-        partials = runtime.GetScratch("coulomb3d.partials", bytes);
-    }
-    // One arrival counter per split tile.
-    void* counters = runtime.GetScratch("coulomb3d.counters", std::max<size_t>(1, tiles - static_cast<size_t>(splitFrom))
-        * sizeof(unsigned int));
+    auto& runtime = CoulombRuntime::Get();
+    void* partials = plan.partialBytes > 0 ? runtime.GetScratch("coulomb3d.partials", plan.partialBytes) : nullptr;
+    void* counters = runtime.GetScratch("coulomb3d.counters", std::max<size_t>(1, plan.counterCount) * sizeof(unsigned int));
     void* arguments[] = {&atoms, &unused, &unused, &unused, &unused, &atomCount, &spacing, &gridSize, &energyGrid, &partials, &counters,
-        &splitFrom};
-    CoulombRuntime::Launch(*kernel, grid, shape.block, arguments, hipStream, kernelMs);
+        &plan.splitFrom};
+    CoulombRuntime::Launch(*plan.kernel, plan.grid, plan.shape.block, arguments, hipStream, kernelMs);
+    return 0;
+    KTT_C_CATCH
+}
+
+int ktt_coulomb3d_launch_plan(int atomCount, int gridSize, int form, const char* config, unsigned int grid[3], unsigned int block[3],
+    int* splitFrom, size_t* scratchBytes)
+{
+    KTT_C_TRY
+    if (atomCount < 0)
+    {
+        return SetLastError("Number of atoms must not be negative");
+    }
+    const Coulomb3dPlan plan = PlanCoulomb3d(atomCount, gridSize, form, config);
+    for (int i = 0; i < 3; ++i)
+    {
+        if (grid != nullptr)
+        {
+            grid[i] = plan.grid[i];
+        }
+        if (block != nullptr)
+        {
+            block[i] = plan.shape.block[i];
+        }
+    }
+    if (splitFrom != nullptr)
+    {
+        *splitFrom = plan.splitFrom;
+    }
+    if (scratchBytes != nullptr)
+    {
+        *scratchBytes = plan.partialBytes + plan.counterCount * sizeof(unsigned int);
+    }
     return 0;
     KTT_C_CATCH
 }

@@ -14,6 +14,8 @@ Workloads (sizes of the direct-summation benchmark configurations):
   batch       1024 systems, atom counts log-uniform in [200, 20000], cubic grids with side
               in {32, 48, 64, 96, 128}, h = 0.5 A, atoms uniform in each box, q U[-1,1].
   dynamic     192^3 grid, 131,072 atoms U[0,96)^3, jittered by N(0, 0.05^2 I) per step.
+  reference_example  the reference CoulombSum3d example's own workload: 256^3 grid, 64 atoms
+              U[0,20)^3, q in [0, 0.5) (the only workload the reference publishes a time for).
 The reference's own examples draw from random_device (CoulombSum3d.cpp:86-98) and are not
 reproducible; these generators replace them on both sides of every comparison.
 """
@@ -175,6 +177,20 @@ def dynamic_initial(atom_count: int = 131072, grid: int = 192, spacing: float = 
 
 DYNAMIC_STEPS = 500
 DYNAMIC_SIGMA = 0.05
+
+
+SEED_REFERENCE_EXAMPLE = 6
+
+
+def reference_example(atom_count: int = 64, grid: int = 256, spacing: float = 0.5,
+                      seed: int = SEED_REFERENCE_EXAMPLE) -> Volume:
+    """The reference CoulombSum3d example's workload (Examples/CoulombSum3d/CoulombSum3d.cpp:67-98):
+    a 256^3 grid at h = 0.5 A and 64 atoms with x, y, z ~ U[0, 20) A and q = U[0, 20) / 40, i.e.
+    q in [0, 0.5), drawn from this module's seeded stream instead of random_device."""
+    values = uniform01(seed, 4 * atom_count).reshape(atom_count, 4) * np.float32(20.0)
+    atoms = values.astype(np.float32)
+    atoms[:, 3] = values[:, 3] / np.float32(40.0)
+    return Volume(atoms, spacing, grid)
 
 
 def kat_atoms(count: int = 64) -> np.ndarray:

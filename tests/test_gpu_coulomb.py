@@ -377,23 +377,35 @@ def test_translation_and_axis_swap(capi):
 
 
 def test_compensated_tail_split(capi):
-    """160^3 with the compensated default (64-thread spans of 128 points, 16 rows): 3,200 tiles,
-    more than the workgroups 256 CUs hold at once, so only the tiles of the last, partial round
-    split their atoms and meet in FP64 partials; the others store directly. The last z-planes
-    (split tiles) and sampled points everywhere meet the bar, a second launch is bitwise equal
-    (the merge order is fixed), and TAIL_SPLIT=0 (no split) agrees to twice the bar."""
+    """The compensated default splits only the tiles of the last, partial round of workgroups over
+    atom blocks; those meet in FP64 partials, the others store directly. The grid side is the first
+    of 160..256 whose tile count leaves this device (at the kernel's occupancy) a partial round to
+    split, as the launch plan reports. Points of the split tiles and sampled points everywhere meet
+    the bar, a second launch is bitwise equal (the merge order is fixed), and TAIL_SPLIT=0 (no
+    split) agrees on the split tiles to twice the bar."""
     atoms = workloads.uniform_atoms(30, 600, box=80.0)
-    grid, spacing = 160, 0.5
+    spacing = 0.5
+    for grid in (160, 176, 192, 208, 224, 240, 256):
+        plan = capi.coulomb3d_launch_plan(atoms.shape[0], grid, form=capi.COMPENSATED)
+        if plan["grid"][2] > 1:
+            break
+    print(f"tail split plan at {grid}^3:", plan, flush=True)
+    # The split path must run: more than one atom block, and some (not all) tiles split.
+    assert plan["grid"][2] > 1 and 0 < plan["split_from"] < plan["tiles"] and plan["scratch_bytes"] > 0
+    assert capi.coulomb3d_launch_plan(atoms.shape[0], grid, form=capi.COMPENSATED, config="TAIL_SPLIT=0")["grid"][2] == 1
     first = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED)
     second = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED)
     np.testing.assert_array_equal(first, second)
-    z, y, x = np.meshgrid(np.arange(grid - 2, grid), np.arange(grid), np.arange(grid), indexing="ij")
-    last_planes = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
-    for points in (last_planes, workloads.sample_points(31, 4096, grid)):
+    # Rows (z * G + y) of the split tiles: from the first split tile's row group to the end.
+    rows_per_group = (grid * grid) // plan["grid"][1]
+    first_row = (plan["split_from"] // plan["grid"][0]) * rows_per_group
+    rows = first_row + (workloads.splitmix64(32, 4096, stream=7) % np.uint64(grid * grid - first_row)).astype(np.int64)
+    xs = (workloads.splitmix64(33, 4096, stream=7) % np.uint64(grid)).astype(np.int64)
+    split_points = np.stack([xs, rows % grid, rows // grid], axis=1).astype(np.int32)
+    for points in (split_points, workloads.sample_points(31, 4096, grid)):
         worst, _ = normalised(first, atoms, spacing, points)
         assert worst <= COMPENSATED_BAR, worst
     whole = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED, config="TAIL_SPLIT=0")
-    _, norms = oracle.points_f64(atoms, spacing, last_planes)
-    got = first[last_planes[:, 2], last_planes[:, 1], last_planes[:, 0]].astype(np.float64)
-    reference = whole[last_planes[:, 2], last_planes[:, 1], last_planes[:, 0]]
-    assert (np.abs(got - reference) / norms).max() <= 2 * COMPENSATED_BAR
+    _, norms = oracle.points_f64(atoms, spacing, split_points)
+    index = (split_points[:, 2], split_points[:, 1], split_points[:, 0])
+    assert (np.abs(first[index].astype(np.float64) - whole[index]) / norms).max() <= 2 * COMPENSATED_BAR
