@@ -128,11 +128,18 @@ Features CudaBottlenecks(const Features& f, const ProfileSearcherOptions& option
     return bn;
 }
 
+// Issue slots a single-precision transcendental (v_rsq_f32 and its kind) holds, in units of one
+// v_fma_f32: on MI355X the issue-rate probes (ktt_hip_issue_rates: 8 independent chains per
+// thread, 8 waves per SIMD) run v_fma_f32 at 5.65e13 and v_rsq_f32 at 1.77e13 lane operations
+// per second, and the two do not overlap (a kernel's time is the sum of its FMA-class and
+// transcendental issue times: the Coulomb kernels' counters predict their run time to 1%).
+constexpr double kTransIssueSlots = 3.2;
+
 // The same bottlenecks from CDNA counters. Utilisations of HBM, L2 and vector L1 are bytes
 // moved over the kernel's duration against the peak rates of the options; LDS, occupancy and
 // SIMD activity come from the ratio counters; instruction classes are their share of VALU (or
-// of all) instructions times the VALU busy fraction, transcendentals weighted 4 for their
-// quarter issue rate.
+// of all) instructions times the VALU busy fraction. Transcendentals are their share of the VALU
+// issue time, each weighted kTransIssueSlots.
 Features CdnaBottlenecks(const Features& f, const ProfileSearcherOptions& options)
 {
     Features bn;
@@ -173,7 +180,11 @@ Features CdnaBottlenecks(const Features& f, const ProfileSearcherOptions& option
     const double all = valu + Get(f, "SCALAR_INSTS") + memory + Get(f, "SQ_INSTS_BRANCH");
     bn["bnSP"] = Share(Get(f, "F32_INSTS"), valu) * issue;
     bn["bnDP"] = Share(Get(f, "F64_INSTS"), valu) * issue;
-    bn["bnSFU"] = Clamp01(4.0 * Share(Get(f, "SQ_INSTS_VALU_TRANS_F32"), valu)) * issue;
+    const double trans = std::min(Get(f, "SQ_INSTS_VALU_TRANS_F32"), valu);
+    bn["bnSFU"] = Share(kTransIssueSlots * trans, kTransIssueSlots * trans + (valu - trans)) * issue;
+    // FP32, transcendental, integer and conversion instructions share one VALU issue port on CDNA:
+    // a busy VALU is a bottleneck of their weighted sum, whichever class dominates it.
+    bn["bnVALU"] = issue;
     bn["bnCF"] = Share(Get(f, "SQ_INSTS_BRANCH"), all) * issue;
     bn["bnLDST"] = Share(memory, all) * issue;
     bn["bnTexFu"] = 0.0;
@@ -182,8 +193,11 @@ Features CdnaBottlenecks(const Features& f, const ProfileSearcherOptions& option
     bn["bnBconv"] = Share(Get(f, "SQ_INSTS_VALU_CVT"), valu) * issue;
 
     const double threshold = options.instructionThreshold;
-    const double maxClass = std::max({Share(bn["bnSP"], issue), Share(bn["bnDP"], issue), bn["bnSFU"], Share(bn["bnCF"], issue),
-        Share(bn["bnLDST"], issue), Share(bn["bnInt"], issue), Share(bn["bnMisc"], issue), Share(bn["bnBconv"], issue)});
+    // The VALU's share of all instructions counts as a class of its own: its sub-classes share one
+    // issue port.
+    const double maxClass = std::max({Share(bn["bnSP"], issue), Share(bn["bnDP"], issue), Share(bn["bnSFU"], issue),
+        Share(bn["bnCF"], issue), Share(bn["bnLDST"], issue), Share(bn["bnInt"], issue), Share(bn["bnMisc"], issue),
+        Share(bn["bnBconv"], issue), Share(valu, all)});
     const double weight = maxClass > threshold ? (maxClass - threshold) / (1.0 - threshold) : 0.0;
     bn["bnInstIssue"] = (1.0 - issue) * weight;
     return bn;
@@ -260,6 +274,8 @@ std::map<std::string, double> ProfileBasedSearcher::Features(const ComputationRe
     features["F64_INSTS"] = Get(raw, "SQ_INSTS_VALU_FMA_F64") + Get(raw, "SQ_INSTS_VALU_ADD_F64") + Get(raw, "SQ_INSTS_VALU_MUL_F64");
     features["VMEM_INSTS"] = Get(raw, "SQ_INSTS_VMEM_RD") + Get(raw, "SQ_INSTS_VMEM_WR");
     features["SCALAR_INSTS"] = Get(raw, "SQ_INSTS_SALU") + Get(raw, "SQ_INSTS_SMEM");
+    // VALU issue time in FMA slots: every VALU instruction one, a transcendental kTransIssueSlots.
+    features["VALU_ISSUE_SLOTS"] = Get(raw, "SQ_INSTS_VALU") + (kTransIssueSlots - 1.0) * Get(raw, "SQ_INSTS_VALU_TRANS_F32");
     features["Global size"] = global;
     features["Local size"] = local;
     return features;
@@ -321,6 +337,7 @@ std::map<std::string, double> ProfileBasedSearcher::ComputeChanges(const std::ma
         if (b("bnSP") > r) changes["F32_INSTS"] = -b("bnSP");
         if (b("bnDP") > r) changes["F64_INSTS"] = -b("bnDP");
         if (b("bnSFU") > r) changes["SQ_INSTS_VALU_TRANS_F32"] = -Above(b("bnSFU"), r);
+        if (b("bnVALU") > r) changes["VALU_ISSUE_SLOTS"] = -Above(b("bnVALU"), r);
         if (b("bnCF") > r) changes["SQ_INSTS_BRANCH"] = -Above(b("bnCF"), r);
         if (b("bnLDST") > r) changes["VMEM_INSTS"] = -Above(b("bnLDST"), r);
         if (b("bnInt") > r) changes["SQ_INSTS_VALU_INT32"] = -Above(b("bnInt"), r);

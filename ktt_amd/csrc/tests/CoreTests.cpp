@@ -375,7 +375,7 @@ void TestProfileSearcherAnalysis()
 
     // CDNA counters: 4 GiB of HBM traffic (3:1) in 1 ms is 4.295e12 B/s, 53.7 % of 8 TB/s.
     std::map<std::string, double> cdna{{"FETCH_SIZE", 3.0 * 1024 * 1024}, {"WRITE_SIZE", 1.0 * 1024 * 1024}, {"VALUBusy", 90.0},
-        {"F32_INSTS", 600.0}, {"SQ_INSTS_VALU", 1000.0}, {"SQ_INSTS_VALU_TRANS_F32", 200.0}, {"OccupancyPercent", 40.0},
+        {"F32_INSTS", 600.0}, {"SQ_INSTS_VALU", 1000.0}, {"SQ_INSTS_VALU_TRANS_F32", 600.0}, {"OccupancyPercent", 40.0},
         {"SIMD_UTILIZATION", 0.95}, {"LdsUtil", 30.0}, {"kernel_duration_ns", 1.0e6}, {"Global size", 1.0e6}, {"Local size", 256.0}};
     options.computeUnits = 256;
     bn = ProfileBasedSearcher::AnalyzeBottlenecks(cdna, options);
@@ -383,14 +383,21 @@ void TestProfileSearcherAnalysis()
     CHECK(Near(bn["bnDRAMRead"], 0.75 * hbm));
     CHECK(Near(bn["bnDRAMWrite"], 0.25 * hbm));
     CHECK(Near(bn["bnSP"], 0.6 * 0.9));
-    CHECK(Near(bn["bnSFU"], 0.8 * 0.9));
+    // 600 transcendentals of 1,000 VALU instructions, each holding 3.2 issue slots: 1,920 of 2,320
+    // slots, times the 0.9 VALU busy fraction.
+    const double transShare = 3.2 * 600.0 / (3.2 * 600.0 + 400.0);
+    CHECK(Near(bn["bnSFU"], transShare * 0.9));
     CHECK(Near(bn["bnMPparal"], 0.6));
     CHECK(Near(bn["bnGparal"], 0.05));
     CHECK(Near(bn["bnSMRead"], 0.3));
-    // The transcendental unit, busy 0.72 of the time, is the class above the 0.7 threshold.
-    CHECK(Near(bn["bnInstIssue"], 0.1 * (0.72 - 0.7) / 0.3));
+    // Every instruction is a VALU one: the VALU class (share 1) is above the 0.7 threshold, so the
+    // unused 0.1 of its issue counts fully; the busy VALU asks for fewer weighted VALU slots, and the
+    // transcendentals (0.828 of the issue time) for fewer transcendentals.
+    CHECK(Near(bn["bnVALU"], 0.9));
+    CHECK(Near(bn["bnInstIssue"], 0.1 * (1.0 - 0.7) / 0.3));
     changes = ProfileBasedSearcher::ComputeChanges(bn, cdna, options);
-    CHECK(Near(changes["SQ_INSTS_VALU_TRANS_F32"], -(0.72 - 0.7) / 0.3));
+    CHECK(Near(changes["SQ_INSTS_VALU_TRANS_F32"], -(transShare * 0.9 - 0.7) / 0.3));
+    CHECK(Near(changes["VALU_ISSUE_SLOTS"], -(0.9 - 0.7) / 0.3));
     CHECK(changes.count("F32_INSTS") == 0);  // 0.54 is below the threshold
 
     // Scoring: one counter that should shrink. The candidate that halves it scores +1/3, the
