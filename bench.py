@@ -253,6 +253,53 @@ def slice2d_leg(capi, workloads, args, samples: dict) -> dict:
             * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS}
 
 
+SLICE_STUDY_SHAPES = [
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=256",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=64,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=128",
+    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=64,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=128",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=32,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=128",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=2,ROWS_PER_THREAD=4,ATOM_TILE=64,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=128",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=8,ATOM_TILE=64,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=128",
+]
+# Scaling runs: 32 atom blocks of 64-atom tiles (2,048 workgroups at every size), 1 to 8 tiles per block.
+SLICE_STUDY_SCALING = "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=64,INNER_UNROLL_FACTOR=16,ATOM_BLOCKS=32"
+SLICE_STUDY_ATOMS = (2048, 4096, 8192, 12288, 16384)
+
+
+def slice2d_study(capi, workloads, launches: int = 50) -> dict:
+    """Where the 2D slice's time goes: a few launch shapes at 256^2 x 4,000 atoms, then one launch
+    shape (2,048 workgroups) over atom counts of 1 to 8 tiles per workgroup. A straight line
+    through time against atoms has the launch's fixed cost (ramp, first tile load, final atomic
+    adds) as its intercept and the per-atom cost as its slope."""
+    def time_slice(work, config):
+        atoms = capi.DeviceBuffer.from_array(work.slice_atoms)
+        energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
+        energy.zero()
+        run = lambda: capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy, config=config,
+                                     timed=False)
+        per_launch, _ = timed_loop(capi, run, launches, 2)
+        atoms.close()
+        energy.close()
+        return per_launch * 1e3
+
+    work = workloads.slice2d()
+    shapes = {}
+    for config in SLICE_STUDY_SHAPES:
+        shapes[config] = time_slice(work, config)
+        progress(f"slice study {shapes[config]:.2f} us  {config}")
+    scaling = {}
+    for count in SLICE_STUDY_ATOMS:
+        scaling[count] = time_slice(workloads.slice2d(atom_count=count), SLICE_STUDY_SCALING)
+        progress(f"slice study, 2,048 workgroups, {count} atoms: {scaling[count]:.2f} us")
+    counts = np.array(list(scaling), dtype=np.float64)
+    times = np.array(list(scaling.values()))
+    slope, intercept = np.polyfit(counts, times, 1)
+    return {"workload": "CoulombSum2d 256^2 points, h=0.5 A, atoms U[0,128)^3 with dz^2 to z=31.25 A", "launches": launches,
+            "shapes_us_at_4000_atoms": shapes, "scaling_config": SLICE_STUDY_SCALING, "us_by_atoms": scaling,
+            "fit": {"fixed_us": intercept, "us_per_1000_atoms": slope * 1000.0,
+                    "interactions_per_s_of_the_slope": 65536.0 / (slope * 1e-6)}}
+
+
 def precision_leg(capi, workloads, samples: dict) -> dict:
     work = workloads.clustered()
     n_atoms = work.atoms.shape[0]
@@ -524,6 +571,8 @@ def main() -> int:
     parser.add_argument("--legs", default="all", help="comma-separated legs beside the headline: " + ",".join(LEGS))
     parser.add_argument("--dynamic", action="store_true", help="run the dynamic-tuning workload instead")
     parser.add_argument("--exhaustive", action="store_true", help="with --dynamic: add a whole-space deterministic run")
+    parser.add_argument("--slice2d-study", action="store_true",
+                        help="run only the 2D slice study (launch shapes, fixed and per-atom cost) and print its line")
     parser.add_argument("--dynamic-steps", type=int, default=500, help="steps of the dynamic workload (0: skip its leg)")
     parser.add_argument("--dynamic-budget", type=int, default=48)
     parser.add_argument("--results-prefix", default=None, help="with --dynamic: KTT JSON results files prefix")
@@ -537,6 +586,12 @@ def main() -> int:
         return 2
     if args.dynamic:
         return dynamic_main(args)
+    if args.slice2d_study:
+        from ktt_amd import capi, workloads
+
+        capi.library()
+        print(json.dumps(slice2d_study(capi, workloads)))
+        return 0
 
     from ktt_amd import capi, workloads
 

@@ -157,7 +157,9 @@ int ktt_coulomb3d_slices(const float* atoms, int n_atoms, float spacing, int gri
 
 /* Launch geometry a configuration implies for a grid (for callers that size scratch or report
  * work per workgroup): grid of workgroups and workgroup shape. kind: 3 = CoulombSum3d,
- * 2 = CoulombSum2d / slice kernels (rows = grid_size). */
+ * 2 = CoulombSum2d / slice kernels (rows = grid_size), with the defaults of a slice of more
+ * than 16,384 atoms; z is the configuration's ATOM_BLOCKS, before it is capped at the atom
+ * tiles of a call. */
 int ktt_coulomb_launch_shape(int kind, int grid_size, const char* config, unsigned int grid[3], unsigned int block[3]);
 
 /* ---- 5. Batches of independent systems --------------------------------------------------- */

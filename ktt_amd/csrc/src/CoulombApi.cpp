@@ -382,15 +382,21 @@ static CoulombConfig Defaults3d(const int form, const long gridSize)
     return config;
 }
 
-static CoulombConfig DefaultsSlice()
+static CoulombConfig DefaultsSlice(const int atomCount)
 {
     // One slice (256 x 256 points) has too few points to fill 256 CUs with several waves per
     // SIMD, so the atoms are split over up to 256 blocks (fewer when there are fewer atom tiles)
     // and every thread keeps a small 1 x 4 block: at 256^3 x 65,536 the 256 slice launches run
     // at 72% of the FP32 peak (44% with 16 blocks of 2 x 8 points and no unroll).
+    //
+    // A slice of a few thousand atoms (256^2 x 4,000) is one round of workgroups whose fixed
+    // costs (the tile load at the start, the atomic adds at the end) weigh more: 64-atom tiles
+    // over up to 128 blocks ran 32.7 us against 36.0 us for 128-atom tiles over 32 blocks, the
+    // fastest 64-thread-wide shape of a sweep of 102 on MI355X.
     CoulombConfig config;
+    const bool few = atomCount <= 16384;
     config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "1"}, {"ROWS_PER_THREAD", "4"},
-        {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ATOM_BLOCKS", "256"}};
+        {"ATOM_TILE", few ? "64" : "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ATOM_BLOCKS", few ? "128" : "256"}};
     return config;
 }
 
@@ -759,7 +765,7 @@ int ktt_coulomb_launch_shape(int kind, int gridSize, const char* config, unsigne
     {
         return SetLastError("invalid arguments");
     }
-    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d(KTT_COULOMB_PLAIN, gridSize) : DefaultsSlice());
+    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d(KTT_COULOMB_PLAIN, gridSize) : DefaultsSlice(1 << 30));
     const long rows = kind == 3 ? static_cast<long>(gridSize) * gridSize : gridSize;
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, rows, parsed.Get("ATOM_BLOCKS", 1));
     for (int i = 0; i < 3; ++i)
@@ -987,7 +993,7 @@ int ktt_coulomb2d(const float* atoms, int atomCount, float spacing, int gridX, i
         return SetLastError("grid sizes must be positive");
     }
     CheckGrid(energyGrid, static_cast<long>(gridX) * gridY);
-    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice());
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridX, gridY, atomBlocks);
     // The slice kernel derives the row length and row count from the launch (as the reference
@@ -1066,7 +1072,7 @@ int ktt_coulomb3d_iterative(const float* hostAtoms, int atomCount, float spacing
     }
     CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
     auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice());
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, gridSize, atomBlocks);
     CheckSliceGrid(shape, parsed, gridSize);
@@ -1189,7 +1195,7 @@ int ktt_coulomb3d_slices(const float* atoms, int atomCount, float spacing, int g
     }
     CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
     auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice());
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, gridSize, atomBlocks);
     CheckSliceGrid(shape, parsed, gridSize);
