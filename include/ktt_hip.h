@@ -67,11 +67,14 @@ int ktt_hip_zero(void* device_pointer, size_t bytes, void* stream);
 int ktt_hip_copy(void* device_destination, const void* device_source, size_t bytes, void* stream);
 /* Issue rates of the current device's vector ALU, in lane operations per second (instructions
  * x 64 lanes), from probe kernels of 8 independent chains per thread at 8 waves per SIMD:
- * v_fma_f32 alone, v_rsq_f32 alone, and the Coulomb inner loop's mix of 2 FMA + 1 v_rsq_f32.
+ * v_fma_f32 with VGPR operands (dual issue across waves), v_fma_f32 with an SGPR operand
+ * (single issue), v_rsq_f32 alone, and the Coulomb inner loop's mix of 2 FMA + 1 v_rsq_f32 in
+ * each wave. Each rate is marginal: the probes run `iterations` and 2 x `iterations` rounds and
+ * the rate is the extra work over the extra time, free of the launch's fixed costs.
  * Replaces nothing in the reference: it calibrates the transcendental weight of the CDNA
  * bottleneck analysis (the CUDA one reads the XU pipe's utilisation from CUPTI,
  * Source/Api/Searcher/ProfileBasedSearcher.h:123-340). Each output optional. */
-int ktt_hip_issue_rates(int iterations, double* fma_rate, double* rsq_rate, double* mix_rate);
+int ktt_hip_issue_rates(int iterations, double* fma_rate, double* fma_scalar_rate, double* rsq_rate, double* mix_rate);
 int ktt_hip_synchronize(void);
 int ktt_hip_stream_create(void** stream);
 int ktt_hip_stream_destroy(void* stream);

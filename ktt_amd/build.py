@@ -59,6 +59,7 @@ KERNEL_CHECKS = [
     ("coulomb_util.hip", "kttZeroFill4", None),
     ("coulomb_util.hip", "kttJitterAtoms", None),
     ("coulomb_util.hip", "kttRateFma", None),
+    ("coulomb_util.hip", "kttRateFmaScalar", None),
     ("coulomb_util.hip", "kttRateRsq", None),
     ("coulomb_util.hip", "kttRateMix", None),
     ("CoulombSum3d.hip", "directCoulombSumCompensated",

@@ -39,7 +39,7 @@ SIGNATURES = {
     "ktt_hip_memcpy_dtoh": (_c.c_int, [_vp, _vp, _c.c_size_t]),
     "ktt_hip_zero": (_c.c_int, [_vp, _c.c_size_t, _vp]),
     "ktt_hip_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
-    "ktt_hip_issue_rates": (_c.c_int, [_c.c_int, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "ktt_hip_issue_rates": (_c.c_int, [_c.c_int] + [_c.POINTER(_c.c_double)] * 4),
     "ktt_hip_synchronize": (_c.c_int, []),
     "ktt_hip_stream_create": (_c.c_int, [_c.POINTER(_vp)]),
     "ktt_hip_stream_destroy": (_c.c_int, [_vp]),
@@ -202,11 +202,12 @@ class DeviceBuffer:
 
 
 def issue_rates(iterations: int = 2048) -> dict:
-    """Vector-ALU issue rates of the device (lane operations per second): FMA alone, v_rsq_f32
-    alone, and the Coulomb mix of 2 FMA + 1 v_rsq_f32."""
-    rates = [ctypes.c_double(0.0) for _ in range(3)]
+    """Marginal vector-ALU issue rates of the device (lane operations per second): FMA with VGPR
+    operands (dual issue across waves), FMA with an SGPR operand (single issue), v_rsq_f32 alone,
+    and the Coulomb mix of 2 FMA + 1 v_rsq_f32 inside each wave."""
+    rates = [ctypes.c_double(0.0) for _ in range(4)]
     _check(library().ktt_hip_issue_rates(iterations, *(ctypes.byref(rate) for rate in rates)), "ktt_hip_issue_rates")
-    return {"fma": rates[0].value, "rsq": rates[1].value, "mix": rates[2].value}
+    return {"fma": rates[0].value, "fma_scalar_operand": rates[1].value, "rsq": rates[2].value, "mix": rates[3].value}
 
 
 def synchronize() -> None:

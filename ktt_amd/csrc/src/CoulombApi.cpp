@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <sstream>
 
@@ -346,38 +347,35 @@ void CoulombRuntime::Launch(const HipKernel& kernel, const unsigned int grid[3],
     }
 }
 
-// Tuned defaults (gfx950, 256^3 grid x 65,536 atoms): the plain form keeps 2 x 16 points per
-// thread, the compensated form (three values per point) 4 x 8.
+// Tuned defaults (gfx950, 256^3 grid x 65,536 atoms): both forms keep 2 x 8 points per thread
+// and run the inner loop in issue phases.
 static CoulombConfig Defaults3d(const int form, const long gridSize)
 {
     // The fastest shapes of each form at 256^3 x 65,536 on MI355X (sweeps of workgroup shape,
-    // register block, atom tile and unroll; unroll 16 is 0.5% ahead of 8 in both forms).  The compensated form forms its row brackets in FP64,
-    // once per workgroup and x span, so it prefers a wide span (2 x 128 points); the plain form
-    // prefers one point per thread and 8 rows.  A row length that is not a multiple of the span
-    // narrows the workgroup to 64 threads in x, so that fewer lanes idle past the row's end.
+    // register block, atom tile and unroll). With ISSUE_PHASES=1 (FMA phase and v_rsq_f32 phase
+    // per atom at different wave priorities, so that the FMAs of two waves issue in one
+    // quad-cycle) both forms run 2 x 8 points per thread: 92.8 ms compensated (128 x 2 threads,
+    // 256-atom tiles) and 92.7 ms plain (128 x 4 threads, 128-atom tiles), against 105.2 and
+    // 103.7 ms for the best shapes of the interleaved loop. A row length that is not a multiple
+    // of the span narrows the workgroup to 64 threads in x, so that fewer lanes idle past the
+    // row's end.
     //
     // TAIL_SPLIT (compensated form, one atom block): the tiles of the last, partial round of
     // workgroups split their atoms so that the round fills the GPU (see ktt_coulomb3d).  The
     // plain form has no split: its blocks would add with FP32 atomics, whose order is not fixed.
     CoulombConfig config;
-    long wgx = 128;
+    const long wgx = gridSize % 256 != 0 ? 64 : 128;
     if (form == KTT_COULOMB_COMPENSATED)
     {
-        if (gridSize % 256 != 0)
-        {
-            wgx = 64;
-        }
         config.values = {{"WORK_GROUP_SIZE_X", std::to_string(wgx)}, {"WORK_GROUP_SIZE_Y", "2"}, {"POINTS_X", "2"},
-            {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "256"}, {"INNER_UNROLL_FACTOR", "16"}, {"ATOM_BLOCKS", "1"}, {"TAIL_SPLIT", "1"}};
+            {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "256"}, {"INNER_UNROLL_FACTOR", "16"}, {"ISSUE_PHASES", "1"},
+            {"ATOM_BLOCKS", "1"}, {"TAIL_SPLIT", "1"}};
     }
     else
     {
-        if (gridSize % 128 != 0)
-        {
-            wgx = 64;
-        }
         config.values = {{"WORK_GROUP_SIZE_X", std::to_string(wgx)}, {"WORK_GROUP_SIZE_Y", std::to_string(512 / wgx)},
-            {"POINTS_X", "1"}, {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ATOM_BLOCKS", "1"}};
+            {"POINTS_X", "2"}, {"ROWS_PER_THREAD", "8"}, {"ATOM_TILE", "128"}, {"INNER_UNROLL_FACTOR", "16"},
+            {"ISSUE_PHASES", "1"}, {"ATOM_BLOCKS", "1"}};
     }
     return config;
 }
@@ -392,11 +390,13 @@ static CoulombConfig DefaultsSlice(const int atomCount)
     // A slice of a few thousand atoms (256^2 x 4,000) is one round of workgroups whose fixed
     // costs (the tile load at the start, the atomic adds at the end) weigh more: 64-atom tiles
     // over up to 128 blocks ran 32.7 us against 36.0 us for 128-atom tiles over 32 blocks, the
-    // fastest 64-thread-wide shape of a sweep of 102 on MI355X.
+    // fastest 64-thread-wide shape of a sweep of 102 on MI355X. Issue phases (ISSUE_PHASES=1)
+    // take that slice from 34.7 to 33.0 us.
     CoulombConfig config;
     const bool few = atomCount <= 16384;
     config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "1"}, {"ROWS_PER_THREAD", "4"},
-        {"ATOM_TILE", few ? "64" : "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ATOM_BLOCKS", few ? "128" : "256"}};
+        {"ATOM_TILE", few ? "64" : "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ISSUE_PHASES", "1"},
+        {"ATOM_BLOCKS", few ? "128" : "256"}};
     return config;
 }
 
@@ -654,12 +654,12 @@ int ktt_hip_copy(void* destination, const void* source, size_t bytes, void* stre
     KTT_C_CATCH
 }
 
-int ktt_hip_issue_rates(int iterations, double* fmaRate, double* rsqRate, double* mixRate)
+int ktt_hip_issue_rates(int iterations, double* fmaRate, double* fmaScalarRate, double* rsqRate, double* mixRate)
 {
     KTT_C_TRY
-    if (iterations <= 0)
+    if (iterations <= 0 || iterations > std::numeric_limits<int>::max() / 2)
     {
-        return SetLastError("iterations must be positive");
+        return SetLastError("iterations must be in [1, INT_MAX / 2]");
     }
     auto& runtime = CoulombRuntime::Get();
     int device = 0;
@@ -670,29 +670,38 @@ int ktt_hip_issue_rates(int iterations, double* fmaRate, double* rsqRate, double
     const unsigned int groups = static_cast<unsigned int>(std::max(units, 1)) * 8u;
     const size_t threads = static_cast<size_t>(groups) * 256;
     float* out = static_cast<float*>(runtime.GetScratch("issue_rates.out", threads * sizeof(float)));
-    const char* names[3] = {"kttRateFma", "kttRateRsq", "kttRateMix"};
-    const double perRound[3] = {8.0, 8.0, 24.0};  // VALU instructions per round and thread
-    double* rates[3] = {fmaRate, rsqRate, mixRate};
-    for (int k = 0; k < 3; ++k)
+    const char* names[4] = {"kttRateFma", "kttRateFmaScalar", "kttRateRsq", "kttRateMix"};
+    const double perRound[4] = {8.0, 8.0, 8.0, 24.0};  // VALU instructions per round and thread
+    double* rates[4] = {fmaRate, fmaScalarRate, rsqRate, mixRate};
+    const unsigned int grid[3] = {groups, 1, 1};
+    const unsigned int block[3] = {256, 1, 1};
+    for (int k = 0; k < 4; ++k)
     {
-        auto kernel = runtime.GetKernel("coulomb_util.hip", names[k], {});
-        void* arguments[] = {&out, &iterations};
-        const unsigned int grid[3] = {groups, 1, 1};
-        const unsigned int block[3] = {256, 1, 1};
-        float best = 0.0f;
-        for (int repeat = 0; repeat < 4; ++repeat)  // the first launch warms up; the fastest of the rest counts
+        if (rates[k] == nullptr)
         {
-            float milliseconds = 0.0f;
-            CoulombRuntime::Launch(*kernel, grid, block, arguments, nullptr, &milliseconds);
-            if (repeat > 0 && (best == 0.0f || milliseconds < best))
+            continue;
+        }
+        auto kernel = runtime.GetKernel("coulomb_util.hip", names[k], {});
+        // Fastest of 3 launches (after one that warms up) at n and at 2n rounds; the difference
+        // is n rounds of every thread without the launch's ramp-up and drain.
+        float best[2] = {0.0f, 0.0f};
+        for (int length = 0; length < 2; ++length)
+        {
+            int rounds = iterations * (length + 1);
+            float scale = 0.9999f;
+            void* arguments[] = {&out, &rounds, &scale};
+            for (int repeat = 0; repeat < 4; ++repeat)
             {
-                best = milliseconds;
+                float milliseconds = 0.0f;
+                CoulombRuntime::Launch(*kernel, grid, block, arguments, nullptr, &milliseconds);
+                if (repeat > 0 && (best[length] == 0.0f || milliseconds < best[length]))
+                {
+                    best[length] = milliseconds;
+                }
             }
         }
-        if (rates[k] != nullptr)
-        {
-            *rates[k] = best > 0.0f ? static_cast<double>(threads) * iterations * perRound[k] / (best * 1.0e-3) : 0.0;
-        }
+        const double extra = static_cast<double>(best[1]) - static_cast<double>(best[0]);
+        *rates[k] = extra > 0.0 ? static_cast<double>(threads) * iterations * perRound[k] / (extra * 1.0e-3) : 0.0;
     }
     return 0;
     KTT_C_CATCH

@@ -76,6 +76,7 @@ void AddCoulombDynamicTuningSpace(Tuner& tuner, const KernelId kernel, const Ker
     tuner.AddParameter(kernel, "ROWS_PER_THREAD", std::vector<uint64_t>{4, 8, 16});
     tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{64, 128});
     tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1});
+    tuner.AddParameter(kernel, "ISSUE_PHASES", std::vector<uint64_t>{0, 1});
     tuner.AddParameter(kernel, "ATOM_BLOCKS", std::vector<uint64_t>{1, 2, kMaxAtomBlocks});
 
     tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::X, "WORK_GROUP_SIZE_X",

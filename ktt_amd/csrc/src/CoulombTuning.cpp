@@ -27,6 +27,7 @@ void AddCoulomb3dTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDe
     tuner.AddParameter(kernel, "ROWS_PER_THREAD", std::vector<uint64_t>{4, 8, 16});
     tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{32, 64, 128, 256});
     tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1, 2, 4, 8, 16});
+    tuner.AddParameter(kernel, "ISSUE_PHASES", std::vector<uint64_t>{0, 1, 2});
 
     tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::X, "WORK_GROUP_SIZE_X",
         ModifierAction::Multiply);
@@ -71,6 +72,7 @@ void AddCoulombSliceTuningSpace(Tuner& tuner, const KernelId kernel, const Kerne
     tuner.AddParameter(kernel, "ROWS_PER_THREAD", std::vector<uint64_t>{2, 4, 8});
     tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{32, 64, 128});
     tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1, 8, 16});
+    tuner.AddParameter(kernel, "ISSUE_PHASES", std::vector<uint64_t>{0, 1, 2});
     tuner.AddParameter(kernel, "ATOM_BLOCKS", std::vector<uint64_t>{1, 4, 16, 64, 128, 256});
 
     tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::X, "WORK_GROUP_SIZE_X",

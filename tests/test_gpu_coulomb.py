@@ -92,6 +92,53 @@ def test_compensated_3d_matches_fp64_oracle(capi, config):
     assert worst <= COMPENSATED_BAR, (worst, rms)
 
 
+PHASE_SHAPES = [
+    None,
+    # scalar bracket reads (ROWS_PER_THREAD not a multiple of 4), 3 points per thread, padded tiles
+    "WORK_GROUP_SIZE_X=32,WORK_GROUP_SIZE_Y=2,POINTS_X=3,ROWS_PER_THREAD=2,ATOM_TILE=32,INNER_UNROLL_FACTOR=4",
+]
+
+
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("shape", PHASE_SHAPES)
+def test_issue_phases_are_bit_identical(capi, form, shape):
+    """ISSUE_PHASES changes the order in which a wave issues its instructions, not the arithmetic:
+    every point adds its atoms in the same order with the same operations, so the interleaved
+    loop (0), the phased loop (1) and the phased loop with a one-atom LDS read-ahead (2) give
+    bit-identical grids (one atom block: no atomics)."""
+    atoms = workloads.uniform_atoms(31, 777, box=20.0)
+    grid, spacing = 40, 0.5
+    grids = []
+    for phases in (0, 1, 2):
+        config = f"ISSUE_PHASES={phases}" if shape is None else f"{shape},ISSUE_PHASES={phases}"
+        print(f"form={form} config={config}", flush=True)
+        grids.append(run_volume(capi, atoms, spacing, grid, form=form, config=config))
+    for other in grids[1:]:
+        np.testing.assert_array_equal(other.view(np.uint32), grids[0].view(np.uint32))
+    worst, rms = normalised(grids[1], atoms, spacing, all_points(grid))
+    assert worst <= (COMPENSATED_BAR if form == 1 else PLAIN_BAR), (worst, rms)
+
+
+@pytest.mark.parametrize("shape", [None, "WORK_GROUP_SIZE_X=32,POINTS_X=2,ROWS_PER_THREAD=2,ATOM_TILE=32"])
+def test_slice_issue_phases_are_bit_identical(capi, shape):
+    """The slice kernel with one atom block (plain stores, no atomics): ISSUE_PHASES 0, 1 and 2
+    agree bit for bit on the 256^2 x 4,000 slice."""
+    work = workloads.slice2d()
+    atoms = capi.DeviceBuffer.from_array(work.slice_atoms)
+    energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
+    grids = []
+    for phases in (0, 1, 2):
+        config = f"ATOM_BLOCKS=1,ISSUE_PHASES={phases}" if shape is None else f"{shape},ATOM_BLOCKS=1,ISSUE_PHASES={phases}"
+        print(f"slice config={config}", flush=True)
+        energy.zero()
+        capi.coulomb2d(atoms, work.slice_atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy, config=config)
+        grids.append(energy.download((work.grid_y, work.grid_x)))
+    for other in grids[1:]:
+        np.testing.assert_array_equal(other.view(np.uint32), grids[0].view(np.uint32))
+    cpu = oracle.coulomb2d_f32(work.slice_atoms, work.spacing, work.grid_x, work.grid_y)
+    assert np.max(np.abs(grids[1] - cpu)) <= SIDE_BY_SIDE
+
+
 def test_compensated_repeated_launch_reuses_counters(capi):
     """The per-tile arrival counters must be left at zero by every launch."""
     atoms = workloads.uniform_atoms(23, 700, box=16.0)

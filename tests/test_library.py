@@ -51,13 +51,13 @@ def test_malformed_configuration_string_is_rejected():
 
 def test_launch_shape_follows_configuration():
     grid, block = capi.launch_shape(3, 256, None)
-    # Plain-form defaults: 128 x 4 threads, 1 x 8 points per thread.
+    # Plain-form defaults: 128 x 4 threads, 2 x 8 points per thread.
     assert block == (128, 4, 1)
-    assert grid == (256 // 128, 256 * 256 // 32, 1)
-    # Rows that are not a multiple of 128 points: 64 x 8 threads.
+    assert grid == (256 // 256, 256 * 256 // 32, 1)
+    # Rows that are not a multiple of 256 points: 64 x 8 threads (a 128-point span).
     grid, block = capi.launch_shape(3, 96, None)
     assert block == (64, 8, 1)
-    assert grid == (2, 96 * 96 // 64, 1)
+    assert grid == (1, 96 * 96 // 64, 1)
     grid, block = capi.launch_shape(3, 100, "WORK_GROUP_SIZE_X=32,POINTS_X=2,WORK_GROUP_SIZE_Y=2,ROWS_PER_THREAD=8,ATOM_BLOCKS=3")
     assert block == (32, 2, 1)
     assert grid == (2, (100 * 100 + 15) // 16, 3)
