@@ -380,23 +380,26 @@ static CoulombConfig Defaults3d(const int form, const long gridSize)
     return config;
 }
 
-static CoulombConfig DefaultsSlice(const int atomCount)
+static CoulombConfig DefaultsSlice(const int atomCount, const long gridX, const long gridY)
 {
     // One slice (256 x 256 points) has too few points to fill 256 CUs with several waves per
-    // SIMD, so the atoms are split over up to 256 blocks (fewer when there are fewer atom tiles)
-    // and every thread keeps a small 1 x 4 block: at 256^3 x 65,536 the 256 slice launches run
-    // at 72% of the FP32 peak (44% with 16 blocks of 2 x 8 points and no unroll).
+    // SIMD, so the atoms are split over up to 256 blocks (fewer when there are fewer atom tiles).
+    // With issue phases (ISSUE_PHASES=1) the 256 slice launches of 256^3 x 65,536 run 97.1 ms
+    // (79% of the FP32 peak) with 2 x 8 points per thread and 100.1 ms with 1 x 4; the 2 x 8
+    // block needs rows that are a multiple of 128 points and a multiple of 32 rows, since the
+    // slice kernels take the row length and count from the launch.
     //
     // A slice of a few thousand atoms (256^2 x 4,000) is one round of workgroups whose fixed
     // costs (the tile load at the start, the atomic adds at the end) weigh more: 64-atom tiles
     // over up to 128 blocks ran 32.7 us against 36.0 us for 128-atom tiles over 32 blocks, the
-    // fastest 64-thread-wide shape of a sweep of 102 on MI355X. Issue phases (ISSUE_PHASES=1)
-    // take that slice from 34.7 to 33.0 us.
+    // fastest 64-thread-wide shape of a sweep of 102 on MI355X. Issue phases take that slice
+    // from 34.7 to 33.0 us.
     CoulombConfig config;
     const bool few = atomCount <= 16384;
-    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", "1"}, {"ROWS_PER_THREAD", "4"},
-        {"ATOM_TILE", few ? "64" : "128"}, {"INNER_UNROLL_FACTOR", "16"}, {"ISSUE_PHASES", "1"},
-        {"ATOM_BLOCKS", few ? "128" : "256"}};
+    const bool wide = !few && gridX % 128 == 0 && gridY % 32 == 0;
+    config.values = {{"WORK_GROUP_SIZE_X", "64"}, {"WORK_GROUP_SIZE_Y", "4"}, {"POINTS_X", wide ? "2" : "1"},
+        {"ROWS_PER_THREAD", wide ? "8" : "4"}, {"ATOM_TILE", few ? "64" : "128"}, {"INNER_UNROLL_FACTOR", "16"},
+        {"ISSUE_PHASES", "1"}, {"ATOM_BLOCKS", few ? "128" : "256"}};
     return config;
 }
 
@@ -774,7 +777,7 @@ int ktt_coulomb_launch_shape(int kind, int gridSize, const char* config, unsigne
     {
         return SetLastError("invalid arguments");
     }
-    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d(KTT_COULOMB_PLAIN, gridSize) : DefaultsSlice(1 << 30));
+    const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d(KTT_COULOMB_PLAIN, gridSize) : DefaultsSlice(1 << 30, gridSize, gridSize));
     const long rows = kind == 3 ? static_cast<long>(gridSize) * gridSize : gridSize;
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, rows, parsed.Get("ATOM_BLOCKS", 1));
     for (int i = 0; i < 3; ++i)
@@ -1002,7 +1005,7 @@ int ktt_coulomb2d(const float* atoms, int atomCount, float spacing, int gridX, i
         return SetLastError("grid sizes must be positive");
     }
     CheckGrid(energyGrid, static_cast<long>(gridX) * gridY);
-    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount));
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount, gridX, gridY));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridX, gridY, atomBlocks);
     // The slice kernel derives the row length and row count from the launch (as the reference
@@ -1081,7 +1084,7 @@ int ktt_coulomb3d_iterative(const float* hostAtoms, int atomCount, float spacing
     }
     CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
     auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount));
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount, gridSize, gridSize));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, gridSize, atomBlocks);
     CheckSliceGrid(shape, parsed, gridSize);
@@ -1204,7 +1207,7 @@ int ktt_coulomb3d_slices(const float* atoms, int atomCount, float spacing, int g
     }
     CheckGrid(energyGrid, static_cast<long>(gridSize) * gridSize * gridSize);
     auto& runtime = CoulombRuntime::Get();
-    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount));
+    const CoulombConfig parsed = CoulombConfig::Parse(config, DefaultsSlice(atomCount, gridSize, gridSize));
     const long atomBlocks = EffectiveAtomBlocks(parsed, atomCount);
     const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, gridSize, atomBlocks);
     CheckSliceGrid(shape, parsed, gridSize);
