@@ -452,12 +452,13 @@ def reference_workload_leg(capi, workloads, args, samples: dict) -> dict:
 NBODY_BODIES = 128 * 1024  # Examples/Nbody/Nbody.cpp:60
 NBODY_FLOP_PER_INTERACTION = 20  # the usual N-body count (3 sub, 3 FMA, rsqrt, 3 mul, 3 FMA)
 NBODY_CONFIGS = [
-    "WORK_GROUP_SIZE_X=256,WORK_GROUP_SIZE_Y=1,BODIES_PER_THREAD=2,ATOM_TILE=256,INNER_UNROLL_FACTOR=4",
-    "WORK_GROUP_SIZE_X=256,WORK_GROUP_SIZE_Y=1,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=4",
-    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=4",
-    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=8",
-    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=1,BODIES_PER_THREAD=8,ATOM_TILE=512,INNER_UNROLL_FACTOR=4",
-    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=2,BODIES_PER_THREAD=8,ATOM_TILE=256,INNER_UNROLL_FACTOR=8",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=8,BODIES_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=8,BODIES_PER_THREAD=2,ATOM_TILE=256,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=16,BODIES_PER_THREAD=2,ATOM_TILE=128,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=8,ISSUE_PHASES=1",
+    # the interleaved loop's best shape
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=8,ISSUE_PHASES=0",
+    "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=4,ISSUE_PHASES=1",
 ]
 
 

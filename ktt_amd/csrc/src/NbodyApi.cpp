@@ -16,11 +16,12 @@ namespace ktt
 void AddNbodyTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefinitionId definition, const uint64_t bodies)
 {
     tuner.AddParameter(kernel, "WORK_GROUP_SIZE_X", std::vector<uint64_t>{64, 128, 256});
-    tuner.AddParameter(kernel, "WORK_GROUP_SIZE_Y", std::vector<uint64_t>{1, 2, 4});
+    tuner.AddParameter(kernel, "WORK_GROUP_SIZE_Y", std::vector<uint64_t>{1, 2, 4, 8, 16});
     tuner.AddParameter(kernel, "BODIES_PER_THREAD", std::vector<uint64_t>{1, 2, 4, 8});
     tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{64, 128, 256, 512});
-    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1, 2, 4, 8});
+    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1, 2, 4, 8, 16});
     tuner.AddParameter(kernel, "USE_SOA", std::vector<uint64_t>{0, 1});
+    tuner.AddParameter(kernel, "ISSUE_PHASES", std::vector<uint64_t>{0, 1});
 
     tuner.AddThreadModifier(kernel, {definition}, ModifierType::Local, ModifierDimension::X, "WORK_GROUP_SIZE_X",
         ModifierAction::Multiply);
@@ -46,16 +47,20 @@ void AddNbodyTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefini
 namespace
 {
 
-// The fastest of the shapes measured at 128K bodies on MI355X; n must be a multiple of 1,024.
+// The fastest of the shapes measured at 128K bodies on MI355X (4.64 ms with issue phases; the
+// interleaved loop's best shape, 64 x 4 threads with 256-body tiles, ran 5.78 ms); n must be a
+// multiple of 1,024. Eight source rows per workgroup give each SIMD four waves, enough for the
+// phased loop's FMA phases to find a partner.
 CoulombConfig NbodyDefaults()
 {
     CoulombConfig defaults;
     defaults.Set("WORK_GROUP_SIZE_X", "64");
-    defaults.Set("WORK_GROUP_SIZE_Y", "4");
+    defaults.Set("WORK_GROUP_SIZE_Y", "8");
     defaults.Set("BODIES_PER_THREAD", "4");
-    defaults.Set("ATOM_TILE", "256");
-    defaults.Set("INNER_UNROLL_FACTOR", "8");
+    defaults.Set("ATOM_TILE", "128");
+    defaults.Set("INNER_UNROLL_FACTOR", "16");
     defaults.Set("USE_SOA", "0");
+    defaults.Set("ISSUE_PHASES", "1");
     return defaults;
 }
 

@@ -76,6 +76,24 @@ def test_nbody_kernel_against_fp64(config):
     check(pos, vel, *run(pos, vel, config))
 
 
+@pytest.mark.parametrize("shape", [None, "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=2,BODIES_PER_THREAD=3,ATOM_TILE=64,"
+                                          "INNER_UNROLL_FACTOR=4"])
+def test_nbody_issue_phases_are_bit_identical(shape):
+    """ISSUE_PHASES=1 (a phase of plain VALU work at wave priority 0, then the v_rsq_f32 at
+    priority 3, per source) reorders the issue, not the arithmetic: every target adds its
+    sources in the same order with the same operations as the interleaved loop."""
+    n = 3072 if shape is not None else 4096
+    pos, vel = bodies(n, seed=13)
+    outputs = []
+    for phases in (0, 1):
+        config = f"ISSUE_PHASES={phases}" if shape is None else f"{shape},ISSUE_PHASES={phases}"
+        print("nbody", config, flush=True)
+        outputs.append(run(pos, vel, config))
+    for interleaved, phased in zip(*outputs):
+        np.testing.assert_array_equal(phased.view(np.uint32), interleaved.view(np.uint32))
+    check(pos, vel, *outputs[1])
+
+
 def test_nbody_reference_kernel_against_fp64():
     pos, vel = bodies(2048)
     check(pos, vel, *run(pos, vel, reference=True))
