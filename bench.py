@@ -296,10 +296,16 @@ def slice2d_study(capi, workloads, launches: int = 50) -> dict:
     counts = np.array(list(scaling), dtype=np.float64)
     times = np.array(list(scaling.values()))
     slope, intercept = np.polyfit(counts, times, 1)
+    # The floor under any launch: back-to-back launches of a one-workgroup fill kernel.
+    blank = capi.DeviceBuffer(256)
+    empty_ms, _ = timed_loop(capi, lambda: blank.zero(), launches, 2)
+    blank.close()
+    progress(f"slice study, an empty launch: {empty_ms * 1e3:.2f} us")
     return {"workload": "CoulombSum2d 256^2 points, h=0.5 A, atoms U[0,128)^3 with dz^2 to z=31.25 A", "launches": launches,
             "shapes_us_at_4000_atoms": shapes, "scaling_config": SLICE_STUDY_SCALING, "us_by_atoms": scaling,
             "fit": {"fixed_us": intercept, "us_per_1000_atoms": slope * 1000.0,
-                    "interactions_per_s_of_the_slope": 65536.0 / (slope * 1e-6)}}
+                    "interactions_per_s_of_the_slope": 65536.0 / (slope * 1e-6)},
+            "empty_launch_us": empty_ms * 1e3}
 
 
 def precision_leg(capi, workloads, samples: dict) -> dict:
