@@ -36,9 +36,13 @@ at sampled grid points: max and RMS of |V - V64| / sum_i |q_i| / r_i per workloa
 
 --dynamic runs only the dynamic-tuning workload and prints its own line; --exhaustive adds a
 whole-space deterministic run for the comparison (it writes the exhaustive file above with
---results-prefix).
+--exhaustive-out). --reference-space searches the reference example's own 210-configuration
+tuning space (Z_ITERATIONS and the rest, CoulombSum3d.cpp:139-209) on CoulombSum3dZ.hip at the
+reference example's workload, the search the reference publishes an RTX 2080 Ti time for.
+--slice2d-study times the single slice's launch shapes and fits its fixed and per-atom cost.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--dynamic [--exhaustive]]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--dynamic [--exhaustive]] [--reference-space]
+       [--slice2d-study]
 """
 
 from __future__ import annotations
@@ -508,6 +512,55 @@ def traffic_summary() -> tuple[float | None, str | None]:
     return float(summary["hbm_bytes_per_launch"]), str(TRAFFIC_SUMMARY.relative_to(REPO))
 
 
+def reference_space_main(args) -> int:
+    """The reference example's own tuning space (its 210 CUDA configurations, Z_ITERATIONS and
+    the rest) searched in full by the ktt Deterministic searcher on this library's
+    CoulombSum3dZ.hip, at the reference example's workload (256^3, 64 atoms U[0,20)^3, q in
+    [0, 0.5)): the same space and workload as the reference's published RTX 2080 Ti search.
+    Every configuration is validated (SideBySide 0.01, as the reference does) against the
+    compensated kernel's grid."""
+    import tempfile
+
+    from ktt_amd import capi, reference_space, workloads
+
+    capi.library()
+    work = workloads.reference_example()
+    atoms = capi.DeviceBuffer.from_array(work.atoms)
+    energy = capi.DeviceBuffer(4 * work.grid ** 3)
+    energy.zero()
+    capi.coulomb3d(atoms, work.atoms.shape[0], work.spacing, work.grid, energy, form=capi.COMPENSATED)
+    reference = energy.download((work.grid ** 3,))
+    atoms.close()
+    energy.close()
+    with tempfile.TemporaryDirectory() as directory:
+        path = reference_space.write_script(Path(directory), work.atoms, work.grid, work.spacing, reference=reference,
+                                            threshold=0.01, logging="Info")
+        progress(f"reference space: {capi.tuning_script_space(str(path))} configurations (a line per configuration follows)")
+        summary = capi.run_tuning_script(str(path), output_file=args.results_prefix + "_reference_space"
+                                         if args.results_prefix else None)
+    best_us = summary["best_ms"] * 1e3
+    line = {
+        "metric": "best kernel time over the reference example's tuning space",
+        "value": best_us,
+        "unit": "us",
+        "higher_is_better": False,
+        "workload": "Examples/CoulombSum3d/CoulombSum3d.cpp:67-98 and :139-209: 256^3 grid, h=0.5 A, 64 atoms U[0,20)^3, "
+                    "q U[0,20)/40 (seeded), the CUDA space of 210 configurations, Deterministic search over all of them",
+        "kernel": "ktt_amd/kernels/CoulombSum3dZ.hip",
+        "space": summary["space"],
+        "tested": summary["tested"],
+        "valid": summary["valid"],
+        "best_config": summary["best_config"],
+        "interactions_per_s": work.interactions / (summary["best_ms"] * 1e-3),
+        "frac_of_fp32_peak": work.interactions / (summary["best_ms"] * 1e-3) * FLOP_PER_INTERACTION / PEAK_FP32_FLOPS,
+        "published": {"us": PUBLISHED_US, "device": PUBLISHED_DEVICE},
+        "vs_published": PUBLISHED_US / best_us,
+        "note": "the same tuning space and workload on different hardware",
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def dynamic_main(args) -> int:
     from ktt_amd import capi, workloads
 
@@ -583,6 +636,8 @@ def main() -> int:
     parser.add_argument("--legs", default="all", help="comma-separated legs beside the headline: " + ",".join(LEGS))
     parser.add_argument("--dynamic", action="store_true", help="run the dynamic-tuning workload instead")
     parser.add_argument("--exhaustive", action="store_true", help="with --dynamic: add a whole-space deterministic run")
+    parser.add_argument("--reference-space", action="store_true",
+                        help="search the reference example's 210-configuration space on CoulombSum3dZ.hip and print its line")
     parser.add_argument("--slice2d-study", action="store_true",
                         help="run only the 2D slice study (launch shapes, fixed and per-atom cost) and print its line")
     parser.add_argument("--dynamic-steps", type=int, default=500, help="steps of the dynamic workload (0: skip its leg)")
@@ -604,6 +659,8 @@ def main() -> int:
         capi.library()
         print(json.dumps(slice2d_study(capi, workloads)))
         return 0
+    if args.reference_space:
+        return reference_space_main(args)
 
     from ktt_amd import capi, workloads
 

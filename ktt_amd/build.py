@@ -68,6 +68,10 @@ KERNEL_CHECKS = [
      "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=4,POINTS_X=1,ROWS_PER_THREAD=8,ATOM_TILE=128,INNER_UNROLL_FACTOR=16"),
     ("CoulombSum3d.hip", "directCoulombSumCompensated",
      "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=2,ROWS_PER_THREAD=8,ATOM_TILE=256,INNER_UNROLL_FACTOR=16"),
+    ("CoulombSum3dZ.hip", "directCoulombSum", None),
+    ("CoulombSum3dZ.hip", "directCoulombSum", "Z_ITERATIONS=32,INNER_UNROLL_FACTOR=16,USE_SOA=1"),
+    ("CoulombSum3dZ.hip", "directCoulombSum", "WORK_GROUP_SIZE_X=16,WORK_GROUP_SIZE_Y=4,Z_ITERATIONS=1,USE_CONSTANT_MEMORY=1"),
+    ("CoulombSum3dZ.hip", "directCoulombSum", "Z_ITERATIONS=4,USE_SOA=1,VECTOR_SIZE=4"),
     ("Nbody.hip", "nbody_kernel", None),
     ("Nbody.hip", "nbody_kernel", "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=8,ATOM_TILE=512,INNER_UNROLL_FACTOR=8,USE_SOA=1"),
     ("NbodyReference.hip", "nbody_kernel_reference", None),
