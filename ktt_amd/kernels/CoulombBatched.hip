@@ -20,10 +20,10 @@
 #define BATCH_THREADS 256
 #endif
 #ifndef BUCKET0_WGX
-#define BUCKET0_WGX 16
+#define BUCKET0_WGX 32
 #endif
 #ifndef BUCKET0_PX
-#define BUCKET0_PX 2
+#define BUCKET0_PX 1
 #endif
 #ifndef BUCKET0_ROWS
 #define BUCKET0_ROWS 4
@@ -56,13 +56,13 @@
 #define BUCKET2_TILE 64
 #endif
 #ifndef BUCKET3_WGX
-#define BUCKET3_WGX 32
+#define BUCKET3_WGX 16
 #endif
 #ifndef BUCKET3_PX
 #define BUCKET3_PX 3
 #endif
 #ifndef BUCKET3_ROWS
-#define BUCKET3_ROWS 8
+#define BUCKET3_ROWS 4
 #endif
 #ifndef BUCKET3_TILE
 #define BUCKET3_TILE 128
@@ -89,7 +89,7 @@
 #define BUCKET2_UNROLL 8
 #endif
 #ifndef BUCKET3_UNROLL
-#define BUCKET3_UNROLL 8
+#define BUCKET3_UNROLL 16
 #endif
 #ifndef BUCKET4_UNROLL
 #define BUCKET4_UNROLL 8
