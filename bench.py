@@ -463,7 +463,7 @@ NBODY_BODIES = 128 * 1024  # Examples/Nbody/Nbody.cpp:60
 NBODY_FLOP_PER_INTERACTION = 20  # the usual N-body count (3 sub, 3 FMA, rsqrt, 3 mul, 3 FMA)
 NBODY_CONFIGS = [
     "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=8,BODIES_PER_THREAD=4,ATOM_TILE=128,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
-    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=8,BODIES_PER_THREAD=2,ATOM_TILE=256,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
+    "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=16,BODIES_PER_THREAD=4,ATOM_TILE=64,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
     "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=16,BODIES_PER_THREAD=2,ATOM_TILE=128,INNER_UNROLL_FACTOR=16,ISSUE_PHASES=1",
     "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=4,BODIES_PER_THREAD=4,ATOM_TILE=256,INNER_UNROLL_FACTOR=8,ISSUE_PHASES=1",
     # the interleaved loop's best shape

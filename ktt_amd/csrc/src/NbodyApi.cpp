@@ -1,5 +1,6 @@
 // N-body step on gfx950: direct launches of the tuned and reference kernels and a tuning run
 // through ktt::Tuner (include/ktt_hip.h section 7), the counterpart of Examples/Nbody/Nbody.cpp.
+#include <algorithm>
 #include <cstdio>
 #include <memory>
 #include <vector>
@@ -38,7 +39,8 @@ void AddNbodyTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefini
     tuner.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y", "BODIES_PER_THREAD", "ATOM_TILE"},
         [bodies](const std::vector<uint64_t>& v)
         {
-            const uint64_t lds = v[3] * v[1] * 16 + (v[1] - 1) * 3 * v[2] * v[0] * 4;
+            // The source tiles and the rows' partial accelerations share one LDS buffer.
+            const uint64_t lds = std::max<uint64_t>(v[3] * v[1] * 16, (v[1] - 1) * 3 * v[2] * v[0] * 4);
             return bodies % (v[3] * v[1]) == 0 && lds <= 64 * 1024;
         });
     tuner.AddConstraint(kernel, {"INNER_UNROLL_FACTOR", "ATOM_TILE"}, [](const std::vector<uint64_t>& v) { return v[1] % v[0] == 0; });
@@ -109,7 +111,7 @@ int ktt_nbody_step(const float* bodyInfo, const float* velocity, int n, float ti
             return SetLastError("the number of bodies (" + std::to_string(n) + ") must be a multiple of WORK_GROUP_SIZE_X * "
                 "BODIES_PER_THREAD and of ATOM_TILE * WORK_GROUP_SIZE_Y");
         }
-        if (tile * y * 16 + (y - 1) * 3 * perThread * x * 4 > 64 * 1024)
+        if (std::max(tile * y * 16, (y - 1) * 3 * perThread * x * 4) > 64 * 1024)
         {
             return SetLastError("the configuration needs more than 64 KiB of LDS");
         }

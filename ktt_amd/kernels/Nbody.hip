@@ -63,9 +63,15 @@ extern "C" __global__ __launch_bounds__(KTT_THREADS) void nbody_kernel(float tim
     const float4* __restrict__ oldVel, const float* __restrict__ oldVelX, const float* __restrict__ oldVelY,
     const float* __restrict__ oldVelZ, float4* __restrict__ newVel, float damping, float softeningSqr, int n)
 {
-    __shared__ float4 tile[WORK_GROUP_SIZE_Y][ATOM_TILE];
+    // The source tiles and, once every row has read its last tile, the rows' partial
+    // accelerations share one LDS buffer.
+    constexpr int kTileBytes = WORK_GROUP_SIZE_Y * ATOM_TILE * 16;
+    constexpr int kPartialBytes = (WORK_GROUP_SIZE_Y - 1) * 3 * BODIES_PER_THREAD * WORK_GROUP_SIZE_X * 4;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kTileBytes > kPartialBytes ? kTileBytes : kPartialBytes];
+    float4 (*tile)[ATOM_TILE] = reinterpret_cast<float4 (*)[ATOM_TILE]>(lds);
 #if WORK_GROUP_SIZE_Y > 1
-    __shared__ float partial[WORK_GROUP_SIZE_Y - 1][3][BODIES_PER_THREAD][WORK_GROUP_SIZE_X];
+    float (*partial)[3][BODIES_PER_THREAD][WORK_GROUP_SIZE_X] =
+        reinterpret_cast<float (*)[3][BODIES_PER_THREAD][WORK_GROUP_SIZE_X]>(lds);
 #endif
 
     const int tx = threadIdx.x;
