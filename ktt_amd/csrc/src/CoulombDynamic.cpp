@@ -67,7 +67,10 @@ uint64_t CeilDiv(const uint64_t value, const uint64_t divisor)
 
 /** Tuning space of the dynamic run: the compensated 3D kernel with the atom range split over
  *  ATOM_BLOCKS workgroups per tile (FP64 merge of the block partials). Spans along x that waste
- *  more than an eighth of a row are excluded. */
+ *  more than an eighth of a row are excluded. The atom loop is unrolled 16 times in every
+ *  configuration: at 192^3 x 131,072 the space's best shape runs 81.7 ms unrolled 16 times,
+ *  81.9 ms 8 times and 84.4 ms not unrolled, and a fixed unroll keeps the space small enough
+ *  for 48 online steps. */
 void AddCoulombDynamicTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefinitionId definition, const uint64_t gridSize)
 {
     tuner.AddParameter(kernel, "WORK_GROUP_SIZE_X", std::vector<uint64_t>{32, 64});
@@ -75,7 +78,7 @@ void AddCoulombDynamicTuningSpace(Tuner& tuner, const KernelId kernel, const Ker
     tuner.AddParameter(kernel, "POINTS_X", std::vector<uint64_t>{1, 2, 3, 4, 6});
     tuner.AddParameter(kernel, "ROWS_PER_THREAD", std::vector<uint64_t>{4, 8, 16});
     tuner.AddParameter(kernel, "ATOM_TILE", std::vector<uint64_t>{64, 128});
-    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{1});
+    tuner.AddParameter(kernel, "INNER_UNROLL_FACTOR", std::vector<uint64_t>{16});
     tuner.AddParameter(kernel, "ISSUE_PHASES", std::vector<uint64_t>{0, 1});
     tuner.AddParameter(kernel, "ATOM_BLOCKS", std::vector<uint64_t>{1, 2, kMaxAtomBlocks});
 
