@@ -49,17 +49,19 @@ void AddNbodyTuningSpace(Tuner& tuner, const KernelId kernel, const KernelDefini
 namespace
 {
 
-// The fastest of the shapes measured at 128K bodies on MI355X (4.64 ms with issue phases; the
-// interleaved loop's best shape, 64 x 4 threads with 256-body tiles, ran 5.78 ms); n must be a
-// multiple of 1,024. Eight source rows per workgroup give each SIMD four waves, enough for the
-// phased loop's FMA phases to find a partner.
+// The fastest of the shapes measured at 128K bodies on MI355X, each timed after about 50 ms of
+// steps at its own shape so that the engine clock has settled: 4.06 ms with issue phases (64 x 8
+// threads with 128-body tiles 4.19 ms; the interleaved loop's best shape, 64 x 4 threads with
+// 256-body tiles, 5.75 ms). n must be a multiple of 1,024. Sixteen source rows per workgroup put
+// two workgroups and eight waves on every SIMD, and the phased loop's plain phases then find a
+// partner in 89% of their quad-cycles (84% with eight rows).
 CoulombConfig NbodyDefaults()
 {
     CoulombConfig defaults;
     defaults.Set("WORK_GROUP_SIZE_X", "64");
-    defaults.Set("WORK_GROUP_SIZE_Y", "8");
+    defaults.Set("WORK_GROUP_SIZE_Y", "16");
     defaults.Set("BODIES_PER_THREAD", "4");
-    defaults.Set("ATOM_TILE", "128");
+    defaults.Set("ATOM_TILE", "64");
     defaults.Set("INNER_UNROLL_FACTOR", "16");
     defaults.Set("USE_SOA", "0");
     defaults.Set("ISSUE_PHASES", "1");
