@@ -111,6 +111,15 @@ __device__ __forceinline__ void twoSumInto(float& s, float& c, const float b)
     c += e;
 }
 
+// prior + (s + c) of a two-sum pair, in FP64 and in the order (prior + s) + c. An infinite term
+// (an atom exactly on a grid point: r^2 = 0) makes s infinite and c NaN (the two-sum's error of
+// inf - inf); s, infinite or NaN from then on, is then the IEEE result of the plain sum, as in
+// the plain form. A finite s comes with a finite c.
+__device__ __forceinline__ double addPair(const double prior, const float s, const float c)
+{
+    return __builtin_isfinite(s) ? prior + static_cast<double>(s) + static_cast<double>(c) : prior + static_cast<double>(s);
+}
+
 // Atom range [a0, a1) of atom block `block` out of `blocks`; blocks are whole
 // tiles so that every block but the last is full.
 __device__ __forceinline__ void atomBlockRange(const int numberOfAtoms, const int block, const int blocks, const int tile,
@@ -501,8 +510,7 @@ struct Engine
             {
                 if (x0 + i < rowLength)
                 {
-                    out[x0 + i] = static_cast<float>(static_cast<double>(out[x0 + i]) + static_cast<double>(sum[k][i])
-                        + static_cast<double>(comp[k][i]));
+                    out[x0 + i] = static_cast<float>(addPair(static_cast<double>(out[x0 + i]), sum[k][i], comp[k][i]));
                 }
             }
         }

@@ -127,7 +127,7 @@ __device__ __forceinline__ void volumeCompensated(const AtomSource src, const in
             {
                 const float2 sc = partials[(static_cast<size_t>(b) * splitTiles + slot) * tileStride
                     + (k * PX + i) * E::Threads + tid];
-                total += static_cast<double>(sc.x) + static_cast<double>(sc.y);
+                total += addPair(-0.0, sc.x, sc.y);  // -0.0 + x == x for every x
             }
             if (row < rowsTotal && x0 + i < gridSize)
             {

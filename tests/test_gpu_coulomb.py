@@ -234,6 +234,38 @@ def test_accumulates_into_grid_and_empty_atoms_leave_it(capi):
     np.testing.assert_array_equal(empty, start)
 
 
+@pytest.mark.parametrize("form,config", [(0, None), (0, "ATOM_BLOCKS=3"), (1, None), (1, "ATOM_BLOCKS=3"),
+                                         (1, "POINTS_X=1,ROWS_PER_THREAD=4,ATOM_TILE=32,ATOM_BLOCKS=2")])
+def test_atom_on_a_grid_point(capi, form, config):
+    """An atom exactly on a grid point gives r = 0 there: q / 0 is +inf or -inf, and 0 / 0 is NaN,
+    in the reference generator's arithmetic (Examples/JSON/generate-coulomb-output.c:28) and in
+    both forms. The compensated form's two-sum turns an infinite term into a NaN error term,
+    which must not reach the grid; every other point stays finite and within the usual bars."""
+    grid, spacing = 24, 0.5
+    atoms = workloads.uniform_atoms(29, 150, box=12.0)
+    on_points = {(5, 7, 9): 0.5, (17, 3, 20): -0.25, (11, 11, 2): 0.0}
+    extra = np.array([[x * spacing, y * spacing, z * spacing, q] for (x, y, z), q in on_points.items()], dtype=np.float32)
+    atoms = np.concatenate([atoms[:70], extra, atoms[70:]])
+    print(f"atom on a grid point form={form} config={config}", flush=True)
+    result = run_volume(capi, atoms, spacing, grid, form=form, config=config)
+    cpu = oracle.coulomb3d_f32(atoms, spacing, grid)
+    special = np.zeros(result.shape, dtype=bool)
+    for (x, y, z), q in on_points.items():
+        special[z, y, x] = True
+        if q > 0:
+            assert np.isposinf(result[z, y, x]) and np.isposinf(cpu[z, y, x])
+        elif q < 0:
+            assert np.isneginf(result[z, y, x]) and np.isneginf(cpu[z, y, x])
+        else:
+            assert np.isnan(result[z, y, x]) and np.isnan(cpu[z, y, x])
+    assert np.isfinite(result[~special]).all()
+    assert np.max(np.abs(result[~special] - cpu[~special])) <= SIDE_BY_SIDE
+    points = all_points(grid)
+    points = points[~special[points[:, 2], points[:, 1], points[:, 0]]]
+    worst, rms = normalised(result, atoms, spacing, points)
+    assert worst <= (COMPENSATED_BAR if form == 1 else PLAIN_BAR), (worst, rms)
+
+
 def test_known_answer_grid_of_reference_generator(capi):
     """Examples/JSON/generate-coulomb-output.c grid (64 atoms, 256^3, h = 0.49) at its sampled points."""
     fixture = json.loads((GOLDEN / "coulomb3d_kat.json").read_text())
