@@ -10,7 +10,9 @@
 //            tunes with the profile-based searcher and saves the results (with counters);
 //        ProfiledCoulombSum3d profile-all [max configurations, 0 = all] [grid side] [atoms] [results file]
 //            runs every configuration in index order with profiling on (every result with counters).
-//        A seventh argument "reduced" limits both to the shapes with INNER_UNROLL_FACTOR 1 and USE_SOA 0.
+//        A seventh argument "reduced" limits both to the 288 shapes with at most 64 threads in x,
+//        tiles of at most 128 atoms, INNER_UNROLL_FACTOR 1, USE_SOA 0 and the interleaved inner loop
+//        (ISSUE_PHASES 0); "reduced-phases" takes the same shapes with ISSUE_PHASES 0 and 1 (576).
 // Counters are enabled before the tuner initialises the HIP runtime.
 #include <cstdio>
 #include <memory>
@@ -109,12 +111,14 @@ int main(int argc, char** argv)
     const auto gridArgument = tuner.AddArgumentVector(p.grid, ktt::ArgumentAccessType::ReadWrite);
     ktt::AddCoulomb3dTuningSpace(tuner, kernel, definition);
     tuner.AddParameter(kernel, "USE_SOA", std::vector<uint64_t>{0, 1});
-    if (search && argc > 7 && std::string(argv[7]) == "reduced")
+    if (search && argc > 7 && (std::string(argv[7]) == "reduced" || std::string(argv[7]) == "reduced-phases"))
     {
         // The 288 shapes with at most 64 threads in x, tiles of at most 128 atoms, no unroll,
-        // float4 atoms and the compiler's interleaved inner loop (of 15,288).
+        // float4 atoms and the compiler's interleaved inner loop (of 15,288); with "reduced-phases"
+        // each of them also with the phased inner loop.
+        const uint64_t phases = std::string(argv[7]) == "reduced-phases" ? 1 : 0;
         tuner.AddConstraint(kernel, {"WORK_GROUP_SIZE_X", "ATOM_TILE", "INNER_UNROLL_FACTOR", "USE_SOA", "ISSUE_PHASES"},
-            [](const std::vector<uint64_t>& v) { return v[0] <= 64 && v[1] <= 128 && v[2] == 1 && v[3] == 0 && v[4] == 0; });
+            [phases](const std::vector<uint64_t>& v) { return v[0] <= 64 && v[1] <= 128 && v[2] == 1 && v[3] == 0 && v[4] <= phases; });
     }
     tuner.SetArguments(definition, {ai, ax, ay, az, aw, n, h, g, gridArgument});
     tuner.SetArguments(referenceDefinition, {ai, n, h, g, gridArgument});

@@ -80,7 +80,8 @@ public:
      *  the run holds no counter set this searcher understands. */
     static std::map<std::string, double> Features(const ComputationResult& computation);
     /** Bottleneck scores in [0, 1] of a feature set (names as in the reference: bnDRAMRead, bnL2Read,
-     *  bnTex, bnLocal, bnSMRead, bnMPparal, bnGparal, bnThreads, bnSP, bnInstIssue, ...). */
+     *  bnTex, bnLocal, bnSMRead, bnMPparal, bnGparal, bnThreads, bnSP, bnInstIssue, ...; on CDNA
+     *  also bnVALU and, from the dual-issue counters, bnPairing). */
     static std::map<std::string, double> AnalyzeBottlenecks(const std::map<std::string, double>& features,
         const ProfileSearcherOptions& options);
     /** Signed importance of changing each feature: negative asks for less, positive for more. */

@@ -198,6 +198,9 @@ const std::vector<std::string>& HipCounterService::DefaultCounters()
         "SQ_INSTS_BRANCH", "SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_TRANS_F32",
         "SQ_INSTS_VALU_INT32", "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
         "SQ_INSTS_MFMA",
+        // VALU issue (quad-cycles; a transcendental counts two) and the quad-cycles in which two
+        // waves' VALU instructions issued together (gfx950 dual issue)
+        "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_VALU2",
     };
     return counters;
 }

@@ -49,6 +49,7 @@ double CounterValue(const KernelProfilingCounter& counter)
 // Counter sets the analysis understands.
 const char* const kCudaDram = "dram__throughput.avg.pct_of_peak_sustained_elapsed";
 const char* const kCdnaValu = "VALUBusy";
+const char* const kCdnaQuadCycles = "VALU_QUAD_CYCLES";
 
 bool IsCuda(const Features& features) { return features.count(kCudaDram) != 0; }
 bool IsCdna(const Features& features) { return features.count("F32_INSTS") != 0; }
@@ -142,7 +143,9 @@ constexpr double kTransIssueSlots = 3.2;
 // moved over the kernel's duration against the peak rates of the options; LDS, occupancy and
 // SIMD activity come from the ratio counters; instruction classes are their share of VALU (or
 // of all) instructions times the VALU busy fraction. Transcendentals are their share of the VALU
-// issue time, each weighted kTransIssueSlots.
+// issue time, each weighted kTransIssueSlots; with the dual-issue counters (SQ_ACTIVE_INST_VALU,
+// SQ_ACTIVE_INST_VALU2) that time is measured, and the quad-cycles lost to instructions that
+// issued alone form a class of their own (bnPairing), answered by candidates that pair more.
 Features CdnaBottlenecks(const Features& f, const ProfileSearcherOptions& options)
 {
     Features bn;
@@ -184,7 +187,21 @@ Features CdnaBottlenecks(const Features& f, const ProfileSearcherOptions& option
     bn["bnSP"] = Share(Get(f, "F32_INSTS"), valu) * issue;
     bn["bnDP"] = Share(Get(f, "F64_INSTS"), valu) * issue;
     const double trans = std::min(Get(f, "SQ_INSTS_VALU_TRANS_F32"), valu);
-    bn["bnSFU"] = Share(kTransIssueSlots * trans, kTransIssueSlots * trans + (valu - trans)) * issue;
+    const double quadCycles = Get(f, kCdnaQuadCycles);
+    if (quadCycles > 0.0)
+    {
+        // Measured issue time: a transcendental holds its SIMD for two quad-cycles and never pairs;
+        // the other VALU instructions take half a quad-cycle each when every one pairs, so what
+        // the kernel took beyond that was lost to instructions that issued alone.
+        bn["bnSFU"] = Clamp01(Share(2.0 * trans, quadCycles)) * issue;
+        const double paired = 0.5 * (valu - trans) + 2.0 * trans;
+        bn["bnPairing"] = Clamp01(Share(quadCycles - paired, quadCycles)) * issue;
+    }
+    else
+    {
+        bn["bnSFU"] = Share(kTransIssueSlots * trans, kTransIssueSlots * trans + (valu - trans)) * issue;
+        bn["bnPairing"] = 0.0;
+    }
     // FP32, transcendental, integer and conversion instructions share one VALU issue port on CDNA:
     // a busy VALU is a bottleneck of their weighted sum, whichever class dominates it.
     bn["bnVALU"] = issue;
@@ -279,6 +296,13 @@ std::map<std::string, double> ProfileBasedSearcher::Features(const ComputationRe
     features["SCALAR_INSTS"] = Get(raw, "SQ_INSTS_SALU") + Get(raw, "SQ_INSTS_SMEM");
     // VALU issue time in FMA slots: every VALU instruction one, a transcendental kTransIssueSlots.
     features["VALU_ISSUE_SLOTS"] = Get(raw, "SQ_INSTS_VALU") + (kTransIssueSlots - 1.0) * Get(raw, "SQ_INSTS_VALU_TRANS_F32");
+    // With the dual-issue counters, the VALU issue time is measured instead: the quad-cycles
+    // issued less those in which two waves issued together.
+    if (raw.count("SQ_ACTIVE_INST_VALU") != 0 && raw.count("SQ_ACTIVE_INST_VALU2") != 0)
+    {
+        features["SQ_ACTIVE_INST_VALU2"] = Get(raw, "SQ_ACTIVE_INST_VALU2");
+        features[kCdnaQuadCycles] = std::max(0.0, Get(raw, "SQ_ACTIVE_INST_VALU") - Get(raw, "SQ_ACTIVE_INST_VALU2"));
+    }
     features["Global size"] = global;
     features["Local size"] = local;
     return features;
@@ -340,7 +364,10 @@ std::map<std::string, double> ProfileBasedSearcher::ComputeChanges(const std::ma
         if (b("bnSP") > r) changes["F32_INSTS"] = -b("bnSP");
         if (b("bnDP") > r) changes["F64_INSTS"] = -b("bnDP");
         if (b("bnSFU") > r) changes["SQ_INSTS_VALU_TRANS_F32"] = -Above(b("bnSFU"), r);
-        if (b("bnVALU") > r) changes["VALU_ISSUE_SLOTS"] = -Above(b("bnVALU"), r);
+        if (b("bnVALU") > r) changes[features.count(kCdnaQuadCycles) != 0 ? kCdnaQuadCycles : "VALU_ISSUE_SLOTS"] = -Above(b("bnVALU"), r);
+        // Quad-cycles lost to unpaired instructions count whatever the VALU load: a candidate that
+        // pairs more of the same instructions issues them in less time.
+        if (b("bnPairing") > 0.0) changes["SQ_ACTIVE_INST_VALU2"] = b("bnPairing");
         if (b("bnCF") > r) changes["SQ_INSTS_BRANCH"] = -Above(b("bnCF"), r);
         if (b("bnLDST") > r) changes["VMEM_INSTS"] = -Above(b("bnLDST"), r);
         if (b("bnInt") > r) changes["SQ_INSTS_VALU_INT32"] = -Above(b("bnInt"), r);

@@ -399,6 +399,23 @@ void TestProfileSearcherAnalysis()
     CHECK(Near(changes["SQ_INSTS_VALU_TRANS_F32"], -(transShare * 0.9 - 0.7) / 0.3));
     CHECK(Near(changes["VALU_ISSUE_SLOTS"], -(0.9 - 0.7) / 0.3));
     CHECK(changes.count("F32_INSTS") == 0);  // 0.54 is below the threshold
+    CHECK(changes.count("SQ_ACTIVE_INST_VALU2") == 0);  // no dual-issue counters, no pairing class
+
+    // With the dual-issue counters the issue time is measured: 1,000 VALU instructions of which
+    // 300 transcendentals took 1,200 quad-cycles. The transcendentals held 600 of them; 700 plain
+    // instructions would take 350 if all paired, so 250 quad-cycles went to unpaired issue.
+    std::map<std::string, double> measured = cdna;
+    measured["SQ_INSTS_VALU_TRANS_F32"] = 300.0;
+    measured["VALU_QUAD_CYCLES"] = 1200.0;
+    measured["SQ_ACTIVE_INST_VALU2"] = 100.0;
+    bn = ProfileBasedSearcher::AnalyzeBottlenecks(measured, options);
+    CHECK(Near(bn["bnSFU"], 600.0 / 1200.0 * 0.9));
+    CHECK(Near(bn["bnPairing"], 250.0 / 1200.0 * 0.9));
+    changes = ProfileBasedSearcher::ComputeChanges(bn, measured, options);
+    CHECK(Near(changes["VALU_QUAD_CYCLES"], -(0.9 - 0.7) / 0.3));
+    CHECK(changes.count("VALU_ISSUE_SLOTS") == 0);
+    CHECK(Near(changes["SQ_ACTIVE_INST_VALU2"], 250.0 / 1200.0 * 0.9));
+    CHECK(changes.count("SQ_INSTS_VALU_TRANS_F32") == 0);  // 0.45 is below the threshold
 
     // Scoring: one counter that should shrink. The candidate that halves it scores +1/3, the
     // one that doubles it -1/3 (below the -0.25 cut-off), one without counters 0.

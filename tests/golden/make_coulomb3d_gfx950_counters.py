@@ -1,14 +1,16 @@
-"""Writes coulomb3d_gfx950_<grid>x<atoms>_counters.json: a profiled run of every configuration of
-the reduced gfx950 CoulombSum3d space (288 shapes: at most 64 threads in x, atom tiles of at
-most 128, no unroll, float4 atoms) on one MI355X, in the compact table of
+"""Writes coulomb3d_gfx950_<grid>x<atoms>[_<space>]_counters.json: a profiled run of every
+configuration of the reduced gfx950 CoulombSum3d space (288 shapes: at most 64 threads in x, atom
+tiles of at most 128, no unroll, float4 atoms, the interleaved inner loop; with the space
+"phases", the same shapes with ISSUE_PHASES 0 and 1) on one MI355X, in the compact table of
 coulomb3d_2080_counters.json. Each configuration holds the rocprofiler-sdk counters of the HIP
 backend's default set and the kernel's static data (VGPRs, LDS, scratch, occupancy).
 
 The input is the KTT JSON results file that the profiling example writes on the GPU:
 
     ktt_amd/bin/ProfiledCoulombSum3d profile-all 0 <grid> <atoms> <results> "" reduced
+    ktt_amd/bin/ProfiledCoulombSum3d profile-all 0 <grid> <atoms> <results> "" reduced-phases
 
-Usage: python tests/golden/make_coulomb3d_gfx950_counters.py <results.json> <grid> <atoms>
+Usage: python tests/golden/make_coulomb3d_gfx950_counters.py <results.json> <grid> <atoms> [phases]
 """
 
 from __future__ import annotations
@@ -49,7 +51,7 @@ def compact(source: dict) -> dict:
 
 
 def main() -> int:
-    if len(sys.argv) != 4:
+    if len(sys.argv) not in (4, 5):
         print(__doc__)
         return 2
     table = compact(json.loads(Path(sys.argv[1]).read_text()))
@@ -57,7 +59,8 @@ def main() -> int:
     text = json.dumps(table, indent=1)[:-2] + ',\n "Rows": [\n' + ",\n".join(
         json.dumps(row, separators=(",", ":")) for row in rows) + "\n ]\n}\n"
     json.loads(text)
-    target = HERE / f"coulomb3d_gfx950_{sys.argv[2]}x{sys.argv[3]}_counters.json"
+    space = f"_{sys.argv[4]}" if len(sys.argv) == 5 else ""
+    target = HERE / f"coulomb3d_gfx950_{sys.argv[2]}x{sys.argv[3]}{space}_counters.json"
     target.write_text(text)
     print(target.name, len(rows), "rows,", len(table["Counters"]), "counters")
     return 0

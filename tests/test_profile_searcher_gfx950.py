@@ -5,9 +5,9 @@ make_coulomb3d_gfx950_counters.py from ProfiledCoulombSum3d profile-all). The ta
 128^3 grid with 8,192 atoms; the knowledge bases are the same space profiled at other sizes.
 
 Mean steps to the fastest configuration over 100 seeds, as measured: Random 133.7, MCMC 67.8,
-profile-based learning from its own profiled runs 131.5, with the target's own counters as
-knowledge base 100.8, with 128^3 x 1,024 atoms as knowledge base 104.2, with 64^3 x 1,024 atoms
-213.5. A 64^3 launch has too few workgroups to fill 256 CUs, so its occupancy and busy counters
+profile-based learning from its own profiled runs 121.8, with the target's own counters as
+knowledge base 103.0, with 128^3 x 1,024 atoms as knowledge base 105.0, with 64^3 x 1,024 atoms
+224.1. A 64^3 launch has too few workgroups to fill 256 CUs, so its occupancy and busy counters
 rank the shapes for another bottleneck than the target's: a knowledge base must come from a
 launch of the target's grid; fewer atoms are enough."""
 
@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 from ktt_amd import capi
-from profiled_tables import GOLDEN, write_expanded
+from profiled_tables import GOLDEN, expand, write_expanded
 
 SEEDS = range(1, 101)
 CU_COUNT = 256  # MI355X
@@ -84,3 +84,79 @@ def test_every_searcher_finds_the_best_of_the_space(tables):
                                 ("ProfileBased", small_grid)):
         durations = capi.simulate_search(target, searcher, knowledge, seed=5, compute_units=CU_COUNT)
         assert durations.min() == pytest.approx(best_ms), (searcher, knowledge)
+
+
+# The same 288 shapes, each with the interleaved (ISSUE_PHASES=0) and the phased (1) inner loop,
+# profiled with the default counter set that includes the dual-issue counters SQ_ACTIVE_INST_VALU
+# and SQ_ACTIVE_INST_VALU2 (coulomb3d_gfx950_<grid>x<atoms>_phases_counters.json, from
+# ProfiledCoulombSum3d profile-all ... reduced-phases). The 50 fastest of the 576 run the phased
+# loop. Mean steps to the fastest over 100 seeds, as measured: Random 276.9, MCMC 139.0,
+# profile-based learning from its own runs 120.5, with 128^3 x 1,024 atoms as knowledge base
+# 30.5; without the two dual-issue counters 195.4 and 115.3.
+DUAL_ISSUE = ("SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_VALU2")
+PHASES_SEEDS = range(1, 51)
+
+
+def write_table(name: str, directory, drop=()):
+    """The expanded table, without the counters in drop; its path and fastest duration (ms)."""
+    table = json.loads((GOLDEN / name).read_text())
+    if drop:
+        keep = [i for i, counter in enumerate(table["Counters"]) if counter not in drop]
+        fixed = len(table["Parameters"]) + 9
+        table["Rows"] = [row[:fixed] + [row[fixed + i] for i in keep] for row in table["Rows"]]
+        table["Counters"] = [table["Counters"][i] for i in keep]
+    path = directory / name.replace("_counters.json", "_dropped.json" if drop else "_profiled.json")
+    path.write_text(json.dumps(expand(table)))
+    count = len(table["Parameters"])
+    return path, min(row[count + 1] for row in table["Rows"] if row[count] == "Ok") / 1000.0
+
+
+@pytest.fixture(scope="module")
+def phase_tables(tmp_path_factory):
+    directory = tmp_path_factory.mktemp("gfx950_phases")
+    target, best_ms = write_table("coulomb3d_gfx950_128x8192_phases_counters.json", directory)
+    knowledge, _ = write_table("coulomb3d_gfx950_128x1024_phases_counters.json", directory)
+    target_dropped, _ = write_table("coulomb3d_gfx950_128x8192_phases_counters.json", directory, DUAL_ISSUE)
+    knowledge_dropped, _ = write_table("coulomb3d_gfx950_128x1024_phases_counters.json", directory, DUAL_ISSUE)
+    return target, best_ms, knowledge, target_dropped, knowledge_dropped
+
+
+def test_phase_tables_hold_the_dual_issue_counters():
+    """Every configuration ran, validated and was profiled; a v_rsq_f32 counts two issue
+    quad-cycles and any other VALU instruction one (SQ_ACTIVE_INST_VALU = VALU + TRANS), at most
+    half the other instructions issue in a pair (none when a launch has fewer waves than the
+    GPU's 1,024 SIMDs can pair), and the phased loop pairs more than the interleaved loop of the
+    same shape."""
+    for name in ("coulomb3d_gfx950_128x8192_phases_counters.json", "coulomb3d_gfx950_128x1024_phases_counters.json"):
+        table = json.loads((GOLDEN / name).read_text())
+        assert len(table["Rows"]) == 576
+        column = {name: i for i, name in enumerate(table["Columns"])}
+        shapes = {}
+        for row in table["Rows"]:
+            assert row[column["Status"]] == "Ok"
+            valu, trans = row[column["SQ_INSTS_VALU"]], row[column["SQ_INSTS_VALU_TRANS_F32"]]
+            active, paired = row[column["SQ_ACTIVE_INST_VALU"]], row[column["SQ_ACTIVE_INST_VALU2"]]
+            assert active == pytest.approx(valu + trans, rel=1e-3)
+            assert 0 <= paired <= (valu - trans) / 2
+            if row[column["SQ_WAVES"]] >= 4096:
+                assert paired > 0
+            shape = tuple(row[column[key]] for key in ("WORK_GROUP_SIZE_X", "WORK_GROUP_SIZE_Y", "POINTS_X", "ROWS_PER_THREAD",
+                                                       "ATOM_TILE"))
+            shapes.setdefault(shape, {})[row[column["ISSUE_PHASES"]]] = paired / (valu - trans)
+        assert len(shapes) == 288
+        assert np.median([pair[1] / pair[0] for pair in shapes.values() if pair[0] > 0]) > 1.2
+
+
+def test_dual_issue_counters_steer_the_profile_searcher(phase_tables):
+    target, best_ms, knowledge, target_dropped, knowledge_dropped = phase_tables
+
+    def mean(path, searcher, base=None):
+        return float(np.mean([steps_to_best(capi.simulate_search(path, searcher, base, seed=seed, compute_units=CU_COUNT), best_ms)
+                              for seed in PHASES_SEEDS]))
+
+    random = mean(target, "Random")
+    with_pairing = mean(target, "ProfileBased", knowledge)
+    without_pairing = mean(target_dropped, "ProfileBased", knowledge_dropped)
+    print(f"random {random:.1f}, profile-based with dual-issue counters {with_pairing:.1f}, without {without_pairing:.1f}")
+    assert with_pairing < 0.5 * without_pairing
+    assert with_pairing < 0.25 * random
