@@ -68,6 +68,7 @@ TRAFFIC_SUMMARY = REPO / "profiles" / "coulomb3d_compensated_256x65536_traffic.j
 EXHAUSTIVE_SUMMARY = REPO / "profiles" / "coulomb3d_dynamic_192x131072_exhaustive.json"
 SAMPLE_POINTS = 4096          # FP64-checked grid points per 3D workload
 BATCH_SAMPLE_POINTS = 64      # per system of the batch
+HOST_BUFFER_STEPS = 3         # headline steps timed with host atoms in and the host grid out
 # The reference's only published time for this path: CoulombSum3d, 256^3 grid x 64 atoms, best of
 # the 210-configuration CUDA search on an RTX 2080 Ti
 # (Examples/CoulombSum3d/coulomb_2080_full_search_space.json:4,5434).
@@ -208,6 +209,24 @@ def headline_leg(capi, workloads, args, samples: dict):
         capi.synchronize()
         grid = energy.download((work.grid, work.grid, work.grid))
         samples["headline" if name == "compensated" else "headline_plain"] = [sample(work.atoms, work.spacing, points, grid)]
+
+    # The same compensated step with the atoms handed over and the grid returned in host memory,
+    # as the ktt::Tuner path moves its arguments (pageable buffers, hipMemcpy): wall time of
+    # upload + clear + kernel + download. PCIe-inclusive, so never the headline value.
+    capi.synchronize()
+    wall = time.perf_counter()
+    for _ in range(HOST_BUFFER_STEPS):
+        atoms.upload(work.atoms)
+        energy.zero()
+        capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=capi.COMPENSATED, config=args.config, timed=False)
+        energy.download((work.grid, work.grid, work.grid))
+    capi.synchronize()
+    wall_ms = 1e3 * (time.perf_counter() - wall) / HOST_BUFFER_STEPS
+    results["host_buffers"] = {"steps": HOST_BUFFER_STEPS, "wall_ms_per_step": wall_ms,
+                               "value": work.interactions / (wall_ms * 1e-3),
+                               "bytes_moved": int(work.atoms.nbytes + 4 * work.grid ** 3),
+                               "note": "compensated step with host atoms in and the host grid out (pageable memory): "
+                                       "PCIe-inclusive, not the headline value"}
 
     for buffer in (atoms, calibration, energy):
         buffer.close()
@@ -673,7 +692,8 @@ def main() -> int:
     info = capi.device_info()
     samples = {}
     work, results = headline_leg(capi, workloads, args, samples)
-    progress(f"headline: compensated {results['compensated']['ms_per_step']:.2f} ms, plain {results['plain']['ms_per_step']:.2f} ms")
+    progress(f"headline: compensated {results['compensated']['ms_per_step']:.2f} ms, plain {results['plain']['ms_per_step']:.2f} ms, "
+             f"with host buffers {results['host_buffers']['wall_ms_per_step']:.2f} ms")
     extras = {}
     rates = capi.issue_rates()
     # A SIMD issues at most two wave64 VALU instructions per quad-cycle (two waves, neither a
@@ -752,6 +772,7 @@ def main() -> int:
         },
         "plain": results["plain"],
         "compensated": results["compensated"],
+        "host_buffers": results["host_buffers"],
         "roofline": {
             "bound": "mfma",
             "bound_note": "compute-bound: 'mfma' is the compute side of the record's two values; the pipe doing the "
