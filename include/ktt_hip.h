@@ -97,13 +97,17 @@ int ktt_hip_event_elapsed_ms(void* start, void* end, float* milliseconds);
  * atoms: float4[n_atoms] on the device. Config names: WORK_GROUP_SIZE_X, WORK_GROUP_SIZE_Y,
  * POINTS_X, ROWS_PER_THREAD, ATOM_TILE, INNER_UNROLL_FACTOR, ATOM_BLOCKS (atom blocks for every
  * tile), TAIL_SPLIT (compensated form with one atom block: 1, the default, splits the tiles of
- * the last, partial round of workgroups over atom blocks; 0 does not). */
+ * the last, partial round of workgroups over atom blocks; 0 does not). Grids of any side whose
+ * G^2 rows fit an int: when there are more than 65,535 row groups (G above 1,023 with the
+ * compensated defaults, above 1,448 with the plain ones) they fold into the launch's x dimension,
+ * and the point index is 64-bit (the reference kernel's int index overflows above G = 1,290). */
 int ktt_coulomb3d(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                   int form, const char* config, void* stream, float* kernel_ms);
 
 /* The launch ktt_coulomb3d makes for these arguments, without launching: the grid of workgroups
  * (z = atom blocks; with the compensated form's tail split, the blocks the split tiles use), the
- * workgroup shape, the first split tile (tiles are numbered y * grid[0] + x; equal to the tile
+ * workgroup shape, the first split tile (tiles are numbered row group * x groups + x group, which
+ * is y * grid[0] + x unless the row groups fold into x; equal to the tile
  * count when nothing is split) and the device scratch of the split in bytes. Compiles the kernel
  * (the split depends on its occupancy), so it needs the device. Each output optional. Replaces
  * nothing in the reference: the occupancy-driven split has no counterpart there. */
@@ -162,7 +166,8 @@ int ktt_coulomb3d_slices(const float* atoms, int n_atoms, float spacing, int gri
  * work per workgroup): grid of workgroups and workgroup shape. kind: 3 = CoulombSum3d,
  * 2 = CoulombSum2d / slice kernels (rows = grid_size), with the defaults of a slice of more
  * than 16,384 atoms; z is the configuration's ATOM_BLOCKS, before it is capped at the atom
- * tiles of a call. */
+ * tiles of a call. For kind 3, more than 65,535 row groups fold into x: grid[0] is then
+ * fold * x groups and grid[1] the row groups / fold, rounded up. */
 int ktt_coulomb_launch_shape(int kind, int grid_size, const char* config, unsigned int grid[3], unsigned int block[3]);
 
 /* ---- 5. Batches of independent systems --------------------------------------------------- */

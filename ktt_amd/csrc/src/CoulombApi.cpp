@@ -242,7 +242,8 @@ void* CoulombRuntime::GetScratch(const std::string& name, const size_t bytes)
     return entry.first;
 }
 
-CoulombShape CoulombRuntime::RowShape(const CoulombConfig& config, const long spanX, const long rows, const long atomBlocks)
+CoulombShape CoulombRuntime::RowShape(const CoulombConfig& config, const long spanX, const long rows, const long atomBlocks,
+    const bool foldRows)
 {
     const long wgx = config.Get("WORK_GROUP_SIZE_X", 64);
     const long wgy = config.Get("WORK_GROUP_SIZE_Y", 4);
@@ -268,18 +269,29 @@ CoulombShape CoulombRuntime::RowShape(const CoulombConfig& config, const long sp
     const long spanPerGroup = wgx * px;
     const long groupsX = (spanX + spanPerGroup - 1) / spanPerGroup;
     const long groupsY = (rows + rowsPerGroup - 1) / rowsPerGroup;
-    if (groupsX > 0x7fffffffL || groupsY > 65535L * 65535L || atomBlocks > 65535L)
+    if (groupsX > 0x7fffffffL || groupsY > 0x7fffffffL || atomBlocks > 65535L)
     {
         throw KttException("Launch grid exceeds the device limits", ExceptionReason::DeviceLimitsExceeded);
     }
+    long fold = 1;
     if (groupsY > 65535L)
     {
-        throw KttException("Launch grid of " + std::to_string(groupsY) + " row groups exceeds 65535; raise ROWS_PER_THREAD or "
-            "WORK_GROUP_SIZE_Y", ExceptionReason::DeviceLimitsExceeded);
+        if (!foldRows)
+        {
+            throw KttException("Launch grid of " + std::to_string(groupsY) + " row groups exceeds 65535; raise ROWS_PER_THREAD "
+                "or WORK_GROUP_SIZE_Y", ExceptionReason::DeviceLimitsExceeded);
+        }
+        fold = (groupsY + 65534L) / 65535L;
+        if (groupsX * fold > 0x7fffffffL)
+        {
+            throw KttException("Launch grid exceeds the device limits", ExceptionReason::DeviceLimitsExceeded);
+        }
     }
-    shape.grid[0] = static_cast<unsigned int>(groupsX);
-    shape.grid[1] = static_cast<unsigned int>(groupsY);
+    shape.grid[0] = static_cast<unsigned int>(groupsX * fold);
+    shape.grid[1] = static_cast<unsigned int>((groupsY + fold - 1) / fold);
     shape.grid[2] = static_cast<unsigned int>(atomBlocks);
+    shape.groupsX = groupsX;
+    shape.groupsY = groupsY;
     shape.block[0] = static_cast<unsigned int>(wgx);
     shape.block[1] = static_cast<unsigned int>(wgy);
     shape.block[2] = 1;
@@ -779,7 +791,7 @@ int ktt_coulomb_launch_shape(int kind, int gridSize, const char* config, unsigne
     }
     const CoulombConfig parsed = CoulombConfig::Parse(config, kind == 3 ? Defaults3d(KTT_COULOMB_PLAIN, gridSize) : DefaultsSlice(1 << 30, gridSize, gridSize));
     const long rows = kind == 3 ? static_cast<long>(gridSize) * gridSize : gridSize;
-    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, rows, parsed.Get("ATOM_BLOCKS", 1));
+    const CoulombShape shape = CoulombRuntime::RowShape(parsed, gridSize, rows, parsed.Get("ATOM_BLOCKS", 1), kind == 3);
     for (int i = 0; i < 3; ++i)
     {
         grid[i] = shape.grid[i];
@@ -824,7 +836,7 @@ static Coulomb3dPlan PlanCoulomb3d(const int atomCount, const int gridSize, cons
         ShrinkTileToAtoms(plan.config, atomCount);
     }
     const long atomBlocks = EffectiveAtomBlocks(plan.config, atomCount);
-    plan.shape = CoulombRuntime::RowShape(plan.config, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks);
+    plan.shape = CoulombRuntime::RowShape(plan.config, gridSize, static_cast<long>(gridSize) * gridSize, atomBlocks, true);
     const std::vector<std::string> defines = plan.config.Defines(kLaunchOnly);
     auto& runtime = CoulombRuntime::Get();
     plan.grid[0] = plan.shape.grid[0];
@@ -837,7 +849,7 @@ static Coulomb3dPlan PlanCoulomb3d(const int atomCount, const int gridSize, cons
     }
 
     plan.kernel = runtime.GetKernel("CoulombSum3d.hip", "directCoulombSumCompensated", defines);
-    const size_t tiles = static_cast<size_t>(plan.shape.grid[0]) * plan.shape.grid[1];
+    const size_t tiles = static_cast<size_t>(plan.shape.groupsX) * static_cast<size_t>(plan.shape.groupsY);
     // ATOM_BLOCKS > 1 splits every tile.  Otherwise, with TAIL_SPLIT, only the tiles of the last
     // round of workgroups are split: with S resident workgroups per launch (CUs x workgroups per
     // CU at this kernel's occupancy), tiles % S tiles are left for that round, and splitting each

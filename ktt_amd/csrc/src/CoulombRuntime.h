@@ -38,6 +38,8 @@ struct CoulombShape
     long threads;
     long rowsPerGroup;
     long pointsPerThread;
+    long groupsX;  // workgroups along x (tiles = groupsX * groupsY)
+    long groupsY;  // row groups; with a fold, grid[0] = fold * groupsX and grid[1] = ceil(groupsY / fold)
 };
 
 class CoulombRuntime
@@ -52,7 +54,11 @@ public:
     /** Device scratch of at least bytes (grown on demand, zero-filled when allocated). */
     void* GetScratch(const std::string& name, const size_t bytes);
 
-    static CoulombShape RowShape(const CoulombConfig& config, const long spanX, const long rows, const long atomBlocks);
+    /** Launch of `rows` rows of spanX points. A launch holds at most 65,535 row groups in y; with
+     *  foldRows (the whole-volume kernels, kc::volumeGroup) more row groups fold into x, else
+     *  they throw DeviceLimitsExceeded. */
+    static CoulombShape RowShape(const CoulombConfig& config, const long spanX, const long rows, const long atomBlocks,
+        const bool foldRows = false);
     /** Workgroups of `threads` threads that the current device holds at once for this kernel:
      *  compute units x workgroups per compute unit at the kernel's occupancy. */
     static size_t ResidentWorkgroups(const HipKernel& kernel, const int threads);

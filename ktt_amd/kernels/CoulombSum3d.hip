@@ -11,8 +11,9 @@
 //   grid  = (ceil(G / (WORK_GROUP_SIZE_X * POINTS_X)),
 //            ceil(G * G / (WORK_GROUP_SIZE_Y * ROWS_PER_THREAD)),
 //            atom blocks)
-// With more than one atom block the plain kernel adds its partial sums with
-// FP32 atomics; the compensated kernel merges them in FP64 instead.
+// with the row groups folded into x when there are more than 65,535 of them
+// (kc::volumeGroup). With more than one atom block the plain kernel adds its
+// partial sums with FP32 atomics; the compensated kernel merges them in FP64.
 #include "coulomb_volume.h"
 
 using E = kc::DefinedEngine;
@@ -22,11 +23,16 @@ extern "C" __global__ __launch_bounds__(KC_THREADS) void directCoulombSum(const 
     int gridSize, float* energyGrid)
 {
     __shared__ __attribute__((aligned(16))) float shared[E::SharedFloats];
+    const kc::VolumeGroup group = kc::volumeGroup(gridSize, E::SpanX, E::Rows);
+    if (!group.valid)
+    {
+        return;
+    }
     const kc::AtomSource src{atomInfo, atomInfoX, atomInfoY, atomInfoZ, atomInfoW};
     const int tid = threadIdx.y * WORK_GROUP_SIZE_X + threadIdx.x;
     const int rowsTotal = gridSize * gridSize;
-    const int rowBase = blockIdx.y * E::Rows;
-    const int x0 = (blockIdx.x * WORK_GROUP_SIZE_X + threadIdx.x) * POINTS_X;
+    const int rowBase = group.row * E::Rows;
+    const int x0 = (group.x * WORK_GROUP_SIZE_X + threadIdx.x) * POINTS_X;
     const int row0 = rowBase + threadIdx.y * ROWS_PER_THREAD;
 
     int a0, a1;

@@ -120,6 +120,38 @@ __device__ __forceinline__ double addPair(const double prior, const float s, con
     return __builtin_isfinite(s) ? prior + static_cast<double>(s) + static_cast<double>(c) : prior + static_cast<double>(s);
 }
 
+// The (x group, row group) of this workgroup in a whole-volume launch of a G^3
+// grid: groupsX = ceil(G / spanX) groups of x points times groupsY =
+// ceil(G^2 / rowsPerGroup) groups of rows. A launch's y dimension holds at most
+// 65,535 workgroups, so a launch with more row groups folds them into x:
+// gridDim.x = fold * groupsX and row group = blockIdx.y * fold + blockIdx.x /
+// groupsX. A launch with gridDim.x == groupsX has fold 1 and maps blockIdx as it
+// is. Workgroups beyond the groups (past fold * groupsX in x, or in the fold's
+// last, partial run of row groups) are not valid and exit.
+struct VolumeGroup
+{
+    int x;
+    int row;
+    int groupsX;
+    int groupsY;
+    bool valid;
+};
+
+__device__ __forceinline__ VolumeGroup volumeGroup(const int gridSize, const int spanX, const int rowsPerGroup)
+{
+    VolumeGroup g;
+    g.groupsX = (gridSize + spanX - 1) / spanX;
+    const unsigned int rows = static_cast<unsigned int>(gridSize) * static_cast<unsigned int>(gridSize);
+    g.groupsY = static_cast<int>((rows + static_cast<unsigned int>(rowsPerGroup) - 1u) / static_cast<unsigned int>(rowsPerGroup));
+    const int perRow = static_cast<int>(gridDim.x) / g.groupsX;
+    const int fold = perRow > 1 ? perRow : 1;
+    const int bx = static_cast<int>(blockIdx.x);
+    g.x = bx % g.groupsX;
+    g.row = static_cast<int>(blockIdx.y) * fold + bx / g.groupsX;
+    g.valid = bx < fold * g.groupsX && g.row < g.groupsY;
+    return g;
+}
+
 // Atom range [a0, a1) of atom block `block` out of `blocks`; blocks are whole
 // tiles so that every block but the last is full.
 __device__ __forceinline__ void atomBlockRange(const int numberOfAtoms, const int block, const int blocks, const int tile,

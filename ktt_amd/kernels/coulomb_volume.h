@@ -10,7 +10,8 @@
 // workgroup's first x, formed in FP64), a two-sum per point per atom tile, and
 // an FP64 merge of the atom-block partials inside the launch.
 //
-// Tiles are numbered blockIdx.y * gridDim.x + blockIdx.x.  Tiles below
+// Tiles are numbered row group * x groups + x group (kc::volumeGroup; without a
+// fold of the row groups that is blockIdx.y * gridDim.x + blockIdx.x).  Tiles below
 // splitFrom take the whole atom range in block 0 and store directly; their
 // other blocks exit at once.  Tiles from splitFrom on split the atom range over
 // the gridDim.z blocks and meet in partials:
@@ -38,14 +39,20 @@ __device__ __forceinline__ void volumeCompensated(const AtomSource src, const in
     __shared__ unsigned int sLast;
     constexpr int PX = E::PointsX;
     constexpr int ROWS = E::RowsPerThread;
+    const VolumeGroup group = volumeGroup(gridSize, E::SpanX, E::Rows);
+    if (!group.valid)
+    {
+        return;
+    }
     const int tid = threadIdx.y * E::LanesX + threadIdx.x;
     const int rowsTotal = gridSize * gridSize;
-    const int rowBase = blockIdx.y * E::Rows;
-    const int xBase = blockIdx.x * E::SpanX;
+    const int rowBase = group.row * E::Rows;
+    const int xBase = group.x * E::SpanX;
     const int xFirst = threadIdx.x * PX;
     const int x0 = xBase + xFirst;
     const int row0 = rowBase + threadIdx.y * ROWS;
-    const size_t tile = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+    const size_t tiles = static_cast<size_t>(group.groupsX) * static_cast<size_t>(group.groupsY);
+    const size_t tile = static_cast<size_t>(group.row) * group.groupsX + group.x;
     const bool split = gridDim.z > 1 && tile >= static_cast<size_t>(splitFrom);
     if (gridDim.z > 1 && !split && blockIdx.z != 0)
     {
@@ -86,7 +93,7 @@ __device__ __forceinline__ void volumeCompensated(const AtomSource src, const in
 
     // Partials of the split tiles laid out [block][split tile][point-in-thread][thread]
     // so that the lanes of a wave store consecutive float2s.
-    const size_t splitTiles = static_cast<size_t>(gridDim.x) * gridDim.y - static_cast<size_t>(splitFrom);
+    const size_t splitTiles = tiles - static_cast<size_t>(splitFrom);
     const size_t slot = tile - static_cast<size_t>(splitFrom);
     const size_t tileStride = static_cast<size_t>(E::Threads) * E::Points;
     float2* mine = partials + (static_cast<size_t>(blockIdx.z) * splitTiles + slot) * tileStride;

@@ -8,6 +8,7 @@ tolerance (SideBySideComparison, absolute 0.01: Examples/CoulombSum3d/CoulombSum
 
 from __future__ import annotations
 
+import ctypes
 import json
 from pathlib import Path
 
@@ -488,3 +489,76 @@ def test_compensated_tail_split(capi):
     _, norms = oracle.points_f64(atoms, spacing, split_points)
     index = (split_points[:, 2], split_points[:, 1], split_points[:, 0])
     assert (np.abs(first[index].astype(np.float64) - whole[index]) / norms).max() <= 2 * COMPENSATED_BAR
+
+
+def read_points(capi, energy, grid, points):
+    """Values of a device grid at points (M, 3) of x, y, z, read one float at a time (the grid may
+    be larger than host memory should hold twice)."""
+    library = capi.library()
+    out = np.empty(points.shape[0], dtype=np.float32)
+    base = energy.ptr.value
+    for i, (x, y, z) in enumerate(points.astype(np.int64)):
+        offset = 4 * ((int(z) * grid + int(y)) * grid + int(x))
+        status = library.ktt_hip_memcpy_dtoh(out[i:].ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(base + offset), 4)
+        assert status == 0
+    return out
+
+
+def all_positive(capi, energy, grid, chunk=1 << 26):
+    """Whether every point of a device grid holds a positive value, read in chunks of floats."""
+    library = capi.library()
+    total = grid ** 3
+    buffer = np.empty(chunk, dtype=np.float32)
+    for start in range(0, total, chunk):
+        count = min(chunk, total - start)
+        status = library.ktt_hip_memcpy_dtoh(buffer.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(energy.ptr.value + 4 * start),
+                                             4 * count)
+        assert status == 0
+        if not (buffer[:count] > 0).all():
+            return False
+    return True
+
+
+@pytest.mark.parametrize("form,grids,config", [
+    # the compensated defaults (64 x 2 threads, 2 x 8 points): 16 rows per group, 75,625 row groups
+    # at 1,100, fold 2; the first side from 1,100 whose last round of workgroups the tail split
+    # divides over atom blocks on this device, if one is found below 1,300
+    (1, range(1100, 1300, 4), None),
+    # plain, 4 rows per group: 90,000 row groups at 600, fold 2
+    (0, [600], "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=1,POINTS_X=4,ROWS_PER_THREAD=4,ATOM_TILE=64"),
+])
+def test_large_grid_folds_row_groups(capi, form, grids, config):
+    """A grid with more than 65,535 row groups folds them into the launch's x dimension; the
+    points of the last row groups (which only a folded launch reaches) and sampled points
+    everywhere meet the bar, and every point is written once (all positive for positive charges;
+    a point computed twice would be off by its own value). Above 1,024^3 points the point index
+    also passes 2^30 floats (2^32 bytes)."""
+    atoms = workloads.uniform_atoms(35, 600, box=grids[0] * 0.5)
+    atoms[:, 3] = np.abs(atoms[:, 3]) + np.float32(0.01)
+    spacing = 0.5
+    for grid in grids:
+        plan = capi.coulomb3d_launch_plan(atoms.shape[0], grid, form=form, config=config)
+        if plan["grid"][2] > 1:
+            break
+    else:
+        grid = grids[0]
+        plan = capi.coulomb3d_launch_plan(atoms.shape[0], grid, form=form, config=config)
+    print(f"large grid form={form} grid={grid} plan={plan}", flush=True)
+    span = plan["block"][0] * (4 if config else 2)
+    assert plan["grid"][1] <= 65535 and plan["grid"][0] >= 2 * -(-grid // span)
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    try:
+        energy.zero()
+        capi.coulomb3d(atoms_buffer, atoms.shape[0], spacing, grid, energy, form=form, config=config)
+        capi.synchronize()
+        last_rows = np.array([[x, grid - 1 - k, grid - 1] for k in range(3) for x in range(grid)], dtype=np.int32)
+        for points in (last_rows, workloads.sample_points(36, 2048, grid)):
+            reference, norms = oracle.points_f64(atoms, spacing, points)
+            values = read_points(capi, energy, grid, points)
+            worst, rms = oracle.normalised_error(values, reference, norms)
+            assert worst <= (COMPENSATED_BAR if form == 1 else PLAIN_BAR), (worst, rms)
+        assert all_positive(capi, energy, grid)
+    finally:
+        energy.close()
+        atoms_buffer.close()

@@ -7,6 +7,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from ktt_amd import build as native_build
@@ -63,6 +64,47 @@ def test_launch_shape_follows_configuration():
     assert grid == (2, (100 * 100 + 15) // 16, 3)
     with pytest.raises(capi.KttHipError):
         capi.launch_shape(3, 256, "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=32")
+
+
+def fold_cover(grid_side, grid, block, span, rows_per_group):
+    """The tiles (row group * x groups + x group) of the workgroups of a volume launch that do
+    not exit, as the kernels derive them (kc::volumeGroup in ktt_amd/kernels/coulomb_core.h),
+    restated over all workgroups; with the x and row group counts."""
+    groups_x = -(-grid_side // span)
+    groups_y = -(-grid_side * grid_side // rows_per_group)
+    fold = max(1, grid[0] // groups_x)
+    bx = np.arange(grid[0])[None, :]
+    by = np.arange(grid[1])[:, None]
+    row = by * fold + bx // groups_x
+    valid = (bx < fold * groups_x) & (row < groups_y)
+    tiles = (row * groups_x + bx % groups_x)[valid]
+    return groups_x, groups_y, tiles
+
+
+def test_volume_launch_folds_row_groups_beyond_the_y_limit():
+    """More than 65,535 row groups fold into x; every (x group, row group) pair is then covered
+    by exactly one workgroup, and a launch within the limit is not folded."""
+    # Plain defaults at 1,500 (not a multiple of 256): 64 x 8 threads, 2 x 8 points: a 128-point
+    # span and 64 rows per group, 35,157 row groups, no fold.
+    grid, block = capi.launch_shape(3, 1500, None)
+    assert block == (64, 8, 1) and grid == (12, -(-1500 * 1500 // 64), 1)
+    # The compensated defaults' 16 rows per group at 1,100: 75,625 row groups, fold 2.
+    config = "WORK_GROUP_SIZE_X=128,WORK_GROUP_SIZE_Y=2,POINTS_X=2,ROWS_PER_THREAD=8"
+    grid, block = capi.launch_shape(3, 1100, config)
+    groups_x, groups_y, tiles = fold_cover(1100, grid, block, 256, 16)
+    assert (groups_x, groups_y) == (5, 75625)
+    assert grid == (2 * groups_x, -(-groups_y // 2), 1)
+    assert np.array_equal(np.sort(tiles), np.arange(groups_x * groups_y))
+    # A tall grid of narrow groups: 2,000 with 4 rows per group is 1,000,000 row groups, fold 16.
+    config = "WORK_GROUP_SIZE_X=64,WORK_GROUP_SIZE_Y=1,POINTS_X=4,ROWS_PER_THREAD=4"
+    grid, block = capi.launch_shape(3, 2000, config)
+    groups_x, groups_y, tiles = fold_cover(2000, grid, block, 256, 4)
+    fold = -(-groups_y // 65535)
+    assert fold == 16 and grid == (fold * groups_x, -(-groups_y // fold), 1) and grid[1] <= 65535
+    assert np.array_equal(np.sort(tiles), np.arange(groups_x * groups_y))
+    # The slice kernels take their row length from the launch and never fold.
+    grid, _ = capi.launch_shape(2, 1100, config)
+    assert grid[0] == -(-1100 // 256)
 
 
 def test_null_buffers_are_rejected_before_any_launch():
