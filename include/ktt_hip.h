@@ -121,7 +121,8 @@ int ktt_coulomb3d_launch_plan(int n_atoms, int grid_size, int form, const char* 
  * (the tail split's FP64-merged partials and arrival counters) is a static device array of
  * 64 MiB + 64 KiB per device; a launch needing more is rejected. The first call on a device
  * sets up (symbol address, occupancy): make it outside graph capture. Calls that overlap on
- * one device race on the shared scratch. grid_size in [1, 1023]; atoms 16-byte aligned. */
+ * one device race on the shared scratch. grid_size in [1, 46340] (row groups fold as in
+ * ktt_coulomb3d); atoms 16-byte aligned. */
 int ktt_coulomb3d_compensated_noalloc(const float* atoms, int n_atoms, float spacing, int grid_size, float* energy_grid,
                                       void* stream);
 

@@ -307,14 +307,15 @@ def coulomb3d_iterative_reference(atoms: DeviceBuffer, n_atoms: int, spacing: fl
 
 def coulomb3d_launch_plan(n_atoms: int, grid: int, form: int = PLAIN, config=None) -> dict:
     """The launch ktt_coulomb3d makes: workgroup grid (z = atom blocks), workgroup shape, first split
-    tile and the split's scratch bytes."""
+    tile and the split's scratch bytes. "groups" is grid x times grid y, the tile count unless the
+    row groups fold into x (then the last run of row groups holds workgroups that exit)."""
     g, b = _uint3(), _uint3()
     split_from = ctypes.c_int(0)
     scratch = ctypes.c_size_t(0)
     _check(library().ktt_coulomb3d_launch_plan(n_atoms, grid, form, _config(config), g, b, ctypes.byref(split_from),
                                                ctypes.byref(scratch)), "ktt_coulomb3d_launch_plan")
     return {"grid": tuple(g), "block": tuple(b), "split_from": split_from.value, "scratch_bytes": scratch.value,
-            "tiles": g[0] * g[1]}
+            "groups": g[0] * g[1]}
 
 
 def launch_shape(kind: int, grid: int, config=None) -> tuple[tuple[int, int, int], tuple[int, int, int]]:

@@ -186,10 +186,10 @@ extern "C" __attribute__((visibility("default"))) int ktt_noalloc_coulomb3d_comp
     {
         return Fail(message, messageLength, "Atom buffer must be 16-byte aligned (float4 records)");
     }
-    // 16 rows per workgroup and at most 65,535 row groups: G^2 <= 1,048,560.
-    if (gridSize <= 0 || gridSize > 1023)
+    // G^2 rows must fit an int; row groups beyond a launch's 65,535 fold into x (kc::volumeGroup).
+    if (gridSize <= 0 || gridSize > 46340)
     {
-        return Fail(message, messageLength, "grid size must be in [1, 1023] (at most 65,535 groups of 16 rows)");
+        return Fail(message, messageLength, "grid size must be in [1, 46340] (G^2 rows in an int)");
     }
     DeviceState* state = nullptr;
     if (EnsureDevice(state, message, messageLength) != 0)
@@ -241,7 +241,9 @@ extern "C" __attribute__((visibility("default"))) int ktt_noalloc_coulomb3d_comp
         const unsigned int groups = (count + 255) / 256 < 1024 ? (count + 255) / 256 : 1024;
         hipLaunchKernelGGL(kttNoAllocClear, dim3(groups), dim3(256), 0, hipStream, counters, count);
     }
-    const dim3 grid(static_cast<unsigned int>(groupsX), static_cast<unsigned int>(groupsY), static_cast<unsigned int>(blocks));
+    const long fold = (groupsY + 65534) / 65535;
+    const dim3 grid(static_cast<unsigned int>(groupsX * fold), static_cast<unsigned int>((groupsY + fold - 1) / fold),
+        static_cast<unsigned int>(blocks));
     const float4* atomInfo = reinterpret_cast<const float4*>(atoms);
     const int split = static_cast<int>(splitFrom);
     if (wide)

@@ -471,7 +471,7 @@ def test_compensated_tail_split(capi):
             break
     print(f"tail split plan at {grid}^3:", plan, flush=True)
     # The split path must run: more than one atom block, and some (not all) tiles split.
-    assert plan["grid"][2] > 1 and 0 < plan["split_from"] < plan["tiles"] and plan["scratch_bytes"] > 0
+    assert plan["grid"][2] > 1 and 0 < plan["split_from"] < plan["groups"] and plan["scratch_bytes"] > 0
     assert capi.coulomb3d_launch_plan(atoms.shape[0], grid, form=capi.COMPENSATED, config="TAIL_SPLIT=0")["grid"][2] == 1
     first = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED)
     second = run_volume(capi, atoms, spacing, grid, form=capi.COMPENSATED)

@@ -132,7 +132,7 @@ def test_no_alloc_library_loads_and_checks_input_before_the_device():
     assert hasattr(exported, "ktt_noalloc_coulomb3d_compensated")
     library = capi.library()
     dummy = ctypes.c_void_p(16)
-    assert library.ktt_coulomb3d_compensated_noalloc(dummy, 10, 0.5, 1024, dummy, None) != 0
+    assert library.ktt_coulomb3d_compensated_noalloc(dummy, 10, 0.5, 46341, dummy, None) != 0
     assert "grid size" in library.ktt_hip_last_error().decode()
     assert library.ktt_coulomb3d_compensated_noalloc(None, 10, 0.5, 16, dummy, None) != 0
     assert "null" in library.ktt_hip_last_error().decode()

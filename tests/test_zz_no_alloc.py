@@ -9,6 +9,8 @@ by sum_i |q_i| / r_i at every sampled point) and meet that bar against the FP64 
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -57,6 +59,45 @@ def test_no_alloc_rejects_bad_input(capi):
     with pytest.raises(capi.KttHipError, match="grid size"):
         capi.coulomb3d_compensated_noalloc(atoms, 10, 0.5, 0, energy)
     with pytest.raises(capi.KttHipError, match="grid size"):
-        capi.coulomb3d_compensated_noalloc(atoms, 10, 0.5, 1024, energy)
+        capi.coulomb3d_compensated_noalloc(atoms, 10, 0.5, 46341, energy)
     with pytest.raises(capi.KttHipError, match="negative"):
         capi.coulomb3d_compensated_noalloc(atoms, -1, 0.5, 16, energy)
+
+
+def test_no_alloc_large_grid_folds_row_groups(capi):
+    """A grid with more row groups than a launch's y dimension holds (1,108^3: 76,729 groups of 16
+    rows, folded into x): the entry point that allocates nothing agrees with ktt_coulomb3d to the
+    bar at sampled points and at the last rows, which only the folded launch reaches."""
+    grid, spacing = 1108, 0.5
+    atoms = workloads.uniform_atoms(37, 600, box=grid * 0.5)
+    atoms_buffer = capi.DeviceBuffer.from_array(atoms)
+    energy = capi.DeviceBuffer(4 * grid ** 3)
+    points = np.concatenate([workloads.sample_points(38, 2048, grid),
+                             np.array([[x, grid - 1, grid - 1] for x in range(grid)], dtype=np.int32)])
+    offsets = 4 * ((points[:, 2].astype(np.int64) * grid + points[:, 1]) * grid + points[:, 0])
+    library = capi.library()
+
+    def read():
+        out = np.empty(points.shape[0], dtype=np.float32)
+        for i, offset in enumerate(offsets):
+            assert library.ktt_hip_memcpy_dtoh(out[i:].ctypes.data_as(ctypes.c_void_p),
+                                               ctypes.c_void_p(energy.ptr.value + int(offset)), 4) == 0
+        return out
+
+    try:
+        print(f"no-alloc call: grid {grid}, {atoms.shape[0]} atoms", flush=True)
+        energy.zero()
+        capi.coulomb3d(atoms_buffer, atoms.shape[0], spacing, grid, energy, form=capi.COMPENSATED)
+        capi.synchronize()
+        existing = read()
+        energy.zero()
+        capi.coulomb3d_compensated_noalloc(atoms_buffer, atoms.shape[0], spacing, grid, energy)
+        capi.synchronize()
+        no_alloc = read()
+    finally:
+        energy.close()
+        atoms_buffer.close()
+    values, norms = oracle.points_f64(atoms, spacing, points)
+    assert (np.abs(no_alloc.astype(np.float64) - existing) / norms).max() <= COMPENSATED_BAR
+    worst, _ = oracle.normalised_error(no_alloc, values, norms)
+    assert worst <= COMPENSATED_BAR
