@@ -657,7 +657,8 @@ def main() -> int:
     parser.add_argument("--batch-config", default=None, help="per-bucket shapes BUCKETb_WGX=...,... (default: built-in)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-extras", action="store_true", help="headline and CPU baseline only")
-    parser.add_argument("--legs", default="all", help="comma-separated legs beside the headline: " + ",".join(LEGS))
+    parser.add_argument("--legs", default="all", help="comma-separated legs beside the headline: " + ",".join(LEGS)
+                        + " (none: the headline alone)")
     parser.add_argument("--dynamic", action="store_true", help="run the dynamic-tuning workload instead")
     parser.add_argument("--exhaustive", action="store_true", help="with --dynamic: add a whole-space deterministic run")
     parser.add_argument("--reference-space", action="store_true",
@@ -688,6 +689,13 @@ def main() -> int:
 
     from ktt_amd import capi, workloads
 
+    # "none" (or --no-extras) runs the headline alone.
+    legs = set(LEGS) if args.legs == "all" else {leg for leg in args.legs.split(",") if leg and leg != "none"}
+    if args.no_extras:
+        legs = set()
+    unknown = legs - set(LEGS)
+    if unknown:
+        raise SystemExit(f"unknown legs {sorted(unknown)}; choose from {', '.join(LEGS)} or none")
     capi.library()
     info = capi.device_info()
     samples = {}
@@ -714,12 +722,6 @@ def main() -> int:
     }
     progress(f"issue rates: fma {rates['fma']:.3e}, fma with an SGPR operand {rates['fma_scalar_operand']:.3e}, "
              f"rsq {rates['rsq']:.3e}, mix {rates['mix']:.3e} lane-ops/s")
-    legs = set(LEGS) if args.legs == "all" else {leg for leg in args.legs.split(",") if leg}
-    if args.no_extras:
-        legs = set()
-    unknown = legs - set(LEGS)
-    if unknown:
-        raise SystemExit(f"unknown legs {sorted(unknown)}; choose from {', '.join(LEGS)}")
     if "slice2d" in legs:
         extras["slice2d"] = slice2d_leg(capi, workloads, args, samples)
         progress(f"slice2d {extras['slice2d']['ms'] * 1e3:.1f} us")
