@@ -14,6 +14,8 @@ and one v_rsq_f32 per interaction issue in at best 3 quad-cycles per wave, 11/12
 
 Beside the headline, the same line reports (each timed with HIP events around the kernels):
   plain        the plain FP32 form on the same workload;
+  host_buffers the compensated step with host atoms uploaded and the grid downloaded to host
+               memory, wall time (PCIe-inclusive; never the headline value);
   slice2d      CoulombSum2d: one 256^2 slice at z = 31.25 A, 4,000 atoms;
   iterative    CoulombSum3dIterative on the headline inputs: 256 slice launches with the
                reference's host dz^2 precompute and upload per slice, and the fused device form;
@@ -27,7 +29,9 @@ Beside the headline, the same line reports (each timed with HIP events around th
                of a whole-space search (profiles/coulomb3d_dynamic_192x131072_exhaustive.json) is
                re-timed on the same box for the converged / exhaustive ratio;
   reference_workload  the reference example's own workload (256^3, 64 atoms U[0,20)^3,
-               q in [0, 0.5)), the only one with a published time (769.696 us, RTX 2080 Ti).
+               q in [0, 0.5)), the only one with a published time (769.696 us, RTX 2080 Ti);
+  nbody        one step of the N-body sibling kernel at the example's 131,072 bodies, for six
+               launch shapes, each timed after its own clock warm-up.
 
 The CPU baseline leg times the oracle's FP32 restatement of the reference CPU generator
 (Examples/JSON/generate-coulomb-output.c:22-31) on all host cores over a seeded sample of
