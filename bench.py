@@ -12,7 +12,9 @@ comes from the committed rocprofv3 counter summary when it is present. The issue
 holds the vector-ALU issue probes and the bound they imply for this instruction mix: two FMAs
 and one v_rsq_f32 per interaction issue in at best 3 quad-cycles per wave, 11/12 of the peak.
 
-Beside the headline, the same line reports (each timed with HIP events around the kernels):
+Beside the headline, the same line reports (each timed with HIP events around the kernels;
+sub-millisecond launches after 100 ms of untimed launches, so that the engine clock, which falls
+while the GPU idles between legs, has climbed back):
   plain        the plain FP32 form on the same workload;
   host_buffers the compensated step with host atoms uploaded and the grid downloaded to host
                memory, wall time (PCIe-inclusive; never the headline value);
@@ -73,6 +75,7 @@ EXHAUSTIVE_SUMMARY = REPO / "profiles" / "coulomb3d_dynamic_192x131072_exhaustiv
 SAMPLE_POINTS = 4096          # FP64-checked grid points per 3D workload
 BATCH_SAMPLE_POINTS = 64      # per system of the batch
 HOST_BUFFER_STEPS = 3         # headline steps timed with host atoms in and the host grid out
+SHORT_LAUNCH_WARM_MS = 100.0  # untimed work before timing sub-millisecond launches (engine clock)
 # The reference's only published time for this path: CoulombSum3d, 256^3 grid x 64 atoms, best of
 # the 210-configuration CUDA search on an RTX 2080 Ti
 # (Examples/CoulombSum3d/coulomb_2080_full_search_space.json:4,5434).
@@ -169,11 +172,20 @@ def progress(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
 
-def timed_loop(capi, launch, steps: int, warmup: int) -> tuple[float, float]:
-    """(kernel ms per launch from one event pair around the loop, wall ms per launch)."""
+def timed_loop(capi, launch, steps: int, warmup: int, warm_ms: float = 0.0) -> tuple[float, float]:
+    """(kernel ms per launch from one event pair around the loop, wall ms per launch). warmup
+    launches run first, and then more until warm_ms of wall time has passed: the engine clock
+    falls while the GPU idles (between legs, during a compilation) and climbs back over tens of
+    milliseconds of work, which a run of sub-millisecond launches would otherwise be timed on."""
     for _ in range(warmup):
         launch()
     capi.synchronize()
+    if warm_ms > 0.0:
+        start = time.perf_counter()
+        while (time.perf_counter() - start) * 1e3 < warm_ms:
+            for _ in range(16):
+                launch()
+            capi.synchronize()
     start, end = capi.Event(), capi.Event()
     wall = time.perf_counter()
     start.record()
@@ -268,8 +280,8 @@ def slice2d_leg(capi, workloads, args, samples: dict) -> dict:
     energy = capi.DeviceBuffer(4 * work.grid_x * work.grid_y)
     energy.zero()
     per_launch, wall = timed_loop(capi, lambda: capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x,
-                                                               work.grid_y, energy, timed=False), max(args.steps, 20),
-                                  max(args.warmup, 1))
+                                                               work.grid_y, energy, timed=False), max(args.steps, 200),
+                                  max(args.warmup, 1), SHORT_LAUNCH_WARM_MS)
     energy.zero()
     capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy)
     y, x = np.meshgrid(np.arange(work.grid_y), np.arange(work.grid_x), indexing="ij")
@@ -306,7 +318,7 @@ def slice2d_study(capi, workloads, launches: int = 50) -> dict:
         energy.zero()
         run = lambda: capi.coulomb2d(atoms, work.atoms.shape[0], work.spacing, work.grid_x, work.grid_y, energy, config=config,
                                      timed=False)
-        per_launch, _ = timed_loop(capi, run, launches, 2)
+        per_launch, _ = timed_loop(capi, run, launches, 2, SHORT_LAUNCH_WARM_MS)
         atoms.close()
         energy.close()
         return per_launch * 1e3
@@ -325,7 +337,7 @@ def slice2d_study(capi, workloads, launches: int = 50) -> dict:
     slope, intercept = np.polyfit(counts, times, 1)
     # The floor under any launch: back-to-back launches of a one-workgroup fill kernel.
     blank = capi.DeviceBuffer(256)
-    empty_ms, _ = timed_loop(capi, lambda: blank.zero(), launches, 2)
+    empty_ms, _ = timed_loop(capi, lambda: blank.zero(), launches, 2, SHORT_LAUNCH_WARM_MS)
     blank.close()
     progress(f"slice study, an empty launch: {empty_ms * 1e3:.2f} us")
     return {"workload": "CoulombSum2d 256^2 points, h=0.5 A, atoms U[0,128)^3 with dz^2 to z=31.25 A", "launches": launches,
@@ -468,7 +480,7 @@ def reference_workload_leg(capi, workloads, args, samples: dict) -> dict:
     for name, form in (("plain", capi.PLAIN), ("compensated", capi.COMPENSATED)):
         energy.zero()
         launch = lambda form=form: capi.coulomb3d(atoms, n_atoms, work.spacing, work.grid, energy, form=form, timed=False)
-        per_launch, _ = timed_loop(capi, launch, max(args.steps, 20), max(args.warmup, 1))
+        per_launch, _ = timed_loop(capi, launch, max(args.steps, 100), max(args.warmup, 1), SHORT_LAUNCH_WARM_MS)
         energy.zero()
         launch()
         capi.synchronize()
