@@ -4,6 +4,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -437,7 +438,8 @@ HipEngine::HipEngine(const PlatformIndex platformIndex, const DeviceIndex device
     m_OverrideDefaultOptions(false),
     m_GlobalSizeType(GlobalSizeType::CUDA),
     m_GlobalSizeCorrection(false),
-    m_Profiling(false)
+    m_Profiling(false),
+    m_ClockWarmupMs(50.0)
 {
     if (platformIndex != 0)
     {
@@ -467,7 +469,8 @@ HipEngine::HipEngine(const ComputeApiInitializer& initializer, std::vector<Queue
     m_OverrideDefaultOptions(false),
     m_GlobalSizeType(GlobalSizeType::CUDA),
     m_GlobalSizeCorrection(false),
-    m_Profiling(false)
+    m_Profiling(false),
+    m_ClockWarmupMs(50.0)
 {
     if (initializer.GetQueues().empty())
     {
@@ -500,6 +503,64 @@ void HipEngine::Initialize()
     CheckHipError(hipGetDeviceProperties(&m_Properties, m_DeviceIndex), "hipGetDeviceProperties");
     m_Architecture = HipCompiler::DeviceArchitecture(m_DeviceIndex);
     Logger::LogInfo("Initializing HIP engine on device " + HipCompiler::DeviceName(m_DeviceIndex) + " (" + m_Architecture + ")");
+    if (const char* warmup = std::getenv("KTT_HIP_CLOCK_WARMUP_MS"))
+    {
+        char* end = nullptr;
+        const double value = std::strtod(warmup, &end);
+        if (end != warmup && value >= 0.0)
+        {
+            m_ClockWarmupMs = value;
+        }
+    }
+}
+
+// The source of the clock warm-up kernel: two FMA chains per thread, no memory traffic (the store
+// never happens: the chains stay near 1 and 2).
+static const char* const kClockWarmupSource = R"(
+extern "C" __global__ void kttClockWarmup(float* out, int rounds)
+{
+    float a = 1.0f + 1.0e-3f * static_cast<float>(threadIdx.x & 15u);
+    float b = 2.0f + 1.0e-3f * static_cast<float>(threadIdx.x & 7u);
+    for (int i = 0; i < rounds; ++i)
+    {
+        a = fmaf(a, 0.9999f, 1.0e-4f);
+        b = fmaf(b, 0.9999f, 2.0e-4f);
+    }
+    if (a + b == 123456.0f)
+    {
+        out[threadIdx.x] = a;
+    }
+}
+)";
+
+void HipEngine::WarmClock(hipStream_t stream)
+{
+    if (m_ClockWarmupMs <= 0.0)
+    {
+        return;
+    }
+    if (m_WarmupKernel == nullptr)
+    {
+        std::string lowered;
+        Timer timer;
+        timer.Start();
+        const auto code = HipCompiler::Compile(kClockWarmupSource, "ktt_clock_warmup.hip", "kttClockWarmup",
+            HipCompiler::DefaultOptions(m_Architecture), lowered);
+        timer.Stop();
+        m_WarmupKernel = std::make_shared<HipKernel>("kttClockWarmup", code, lowered, timer.GetElapsedTime());
+        CheckHipError(hipMalloc(&m_WarmupOut, 256 * sizeof(float)), "hipMalloc(clock warm-up)");
+    }
+    int rounds = 4096;
+    void* arguments[] = {&m_WarmupOut, &rounds};
+    const unsigned int groups = static_cast<unsigned int>(std::max(m_Properties.multiProcessorCount, 1)) * 4u;
+    const auto start = std::chrono::steady_clock::now();
+    const auto length = std::chrono::duration<double, std::milli>(m_ClockWarmupMs);
+    while (std::chrono::steady_clock::now() - start < length)
+    {
+        CheckHipError(hipModuleLaunchKernel(m_WarmupKernel->GetFunction(), groups, 1, 1, 256, 1, 1, 0, stream, arguments, nullptr),
+            "hipModuleLaunchKernel(clock warm-up)");
+        CheckHipError(hipStreamSynchronize(stream), "hipStreamSynchronize(clock warm-up)");
+    }
 }
 
 HipEngine::~HipEngine()
@@ -517,6 +578,12 @@ HipEngine::~HipEngine()
     }
     ClearBuffers();
     m_KernelCache.clear();
+    m_WarmupKernel.reset();
+    if (m_WarmupOut != nullptr)
+    {
+        IgnoreHipError(hipFree(m_WarmupOut));
+        m_WarmupOut = nullptr;
+    }
     for (auto& [id, stream] : m_Streams)
     {
         if (m_StreamOwned[id])
@@ -619,6 +686,10 @@ ComputeActionId HipEngine::RunKernelAsync(const KernelComputeData& data, const Q
     timer.Start();
     bool compiled = false;
     auto kernel = LoadKernel(data, compiled);
+    if (compiled)
+    {
+        WarmClock(m_Streams[queueId]);
+    }
 
     // Scalars by host pointer, vectors by device-pointer slot, local arguments as dynamic LDS.
     std::vector<void*> devicePointers;

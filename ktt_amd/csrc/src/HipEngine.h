@@ -212,7 +212,15 @@ private:
     std::vector<std::string> m_PlannedCounters;  // counter list of the service's current plan
     std::vector<size_t> m_PassOf;
 
+    // Clock warm-up after a compilation: the engine clock falls while the GPU waits for the
+    // compiler, so before the first launch of a freshly compiled kernel a small arithmetic kernel
+    // runs for this long (KTT_HIP_CLOCK_WARMUP_MS, default 50; 0 turns it off).
+    double m_ClockWarmupMs;
+    std::shared_ptr<HipKernel> m_WarmupKernel;
+    void* m_WarmupOut = nullptr;
+
     void Initialize();
+    void WarmClock(hipStream_t stream);
     ComputationResult RunWithStaticProfile(const KernelComputeData& data, const QueueId queueId);
     void AddStaticCounters(const KernelComputeData& data, const KernelCompilationData& compilation, KernelProfilingData& profiling) const;
     void EnsurePlan(const std::vector<std::string>& counters);
