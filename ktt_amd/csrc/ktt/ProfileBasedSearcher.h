@@ -15,7 +15,11 @@
   * come without a trained model, from saved results that hold counters:
   *   * a knowledge base: KTT JSON results of a profiled search of the same kernel space
   *     (another input size or another GPU), used for the configurations it holds, as the
-  *     reference's exact-mapping variant does (scoreTuningConfigurationsExact);
+  *     reference's exact-mapping variant does (scoreTuningConfigurationsExact). Its CDNA counts
+  *     that grow with launched threads (instructions, quad-cycles, cache requests) are scaled by
+  *     the ratio of a configuration's threads in this search's launch (Searcher::GetLaunchSize)
+  *     to its threads in the base's, so a base profiled on a smaller problem, whose launches
+  *     overhang it by shape, carries over;
   *   * otherwise the configurations profiled so far in this search: a candidate takes the
   *     counters of the nearest profiled configuration (fewest differing parameter values).
   *

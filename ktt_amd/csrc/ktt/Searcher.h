@@ -45,6 +45,11 @@ public:
     uint64_t GetUnexploredConfigurationsCount() const;
     const std::set<uint64_t>& GetExploredIndices() const;
     bool IsInitialized() const;
+    /** Threads one launch of the configuration runs in the kernel's first definition (global size
+     *  times local size after the thread modifiers, in the tuner's size convention); in a replay of
+     *  saved results, the sizes those results recorded. False when it is not known. This helper has
+     *  no counterpart in the reference's Searcher. */
+    bool GetLaunchSize(const KernelConfiguration& configuration, double& threads) const;
 
     void Initialize(const Exploration& space);
     void Reset();

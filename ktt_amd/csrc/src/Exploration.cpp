@@ -360,23 +360,71 @@ KernelConfiguration Exploration::Best() const
     return m_BestDuration != InvalidDuration ? m_Best : Proposal();
 }
 
+bool Exploration::LaunchSize(const KernelConfiguration& configuration, double& threads) const
+{
+    if (m_SavedLaunchSizes)
+    {
+        try
+        {
+            const auto found = m_LaunchSizes.find(IndexOf(configuration));
+            if (found == m_LaunchSizes.end())
+            {
+                return false;
+            }
+            threads = found->second;
+            return true;
+        }
+        catch (const KttException&)
+        {
+            return false;
+        }
+    }
+    if (m_Kernel.GetDefinitions().empty())
+    {
+        return false;
+    }
+    const KernelDefinition& definition = *m_Kernel.GetDefinitions().front();
+    const auto& pairs = configuration.GetPairs();
+    const DimensionVector global = m_Kernel.GetModifiedSize(definition.GetId(), definition.GetGlobalSize(), ModifierType::Global, pairs);
+    const DimensionVector local = m_Kernel.GetModifiedSize(definition.GetId(), definition.GetLocalSize(), ModifierType::Local, pairs);
+    threads = static_cast<double>(global.GetTotalSize()) * static_cast<double>(local.GetTotalSize());
+    return threads > 0.0;
+}
+
+void Exploration::SetLaunchSizes(std::map<uint64_t, double> sizes)
+{
+    m_LaunchSizes = std::move(sizes);
+    m_SavedLaunchSizes = true;
+}
+
 // ---------------------------------------------------------------- replay
 
 std::vector<KernelResult> ReplayResults(Exploration& exploration, const Kernel& kernel, const std::vector<KernelResult>& saved,
     StopCondition* stop)
 {
     std::map<uint64_t, const KernelResult*> byIndex;
+    std::map<uint64_t, double> launchSizes;
     for (const KernelResult& result : saved)
     {
         try
         {
-            byIndex.emplace(exploration.IndexOf(result.GetConfiguration()), &result);
+            const uint64_t index = exploration.IndexOf(result.GetConfiguration());
+            byIndex.emplace(index, &result);
+            if (!result.GetResults().empty())
+            {
+                const ComputationResult& first = result.GetResults().front();
+                launchSizes.emplace(index, static_cast<double>(first.GetGlobalSize().GetTotalSize())
+                    * static_cast<double>(first.GetLocalSize().GetTotalSize()));
+            }
         }
         catch (const KttException&)
         {
             // Saved results outside the current space are never proposed.
         }
     }
+    // The kernel of a replay need not carry the saved runs' thread modifiers: the launch sizes
+    // the searcher sees are those the saved results recorded.
+    exploration.SetLaunchSizes(std::move(launchSizes));
     if (stop != nullptr)
     {
         stop->Initialize(exploration.Size());

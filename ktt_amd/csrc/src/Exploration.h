@@ -65,6 +65,12 @@ public:
     bool Finished() const { return m_Explored.size() >= m_Size || !m_SearcherLive; }
     /** The fastest valid configuration so far (the first proposal before any valid result). */
     KernelConfiguration Best() const;
+    /** Threads one launch of `configuration` runs in the kernel's first definition: its global
+     *  size times its local size after the thread modifiers. After SetLaunchSizes, only the sizes
+     *  given there count. False when the size is not known. */
+    bool LaunchSize(const KernelConfiguration& configuration, double& threads) const;
+    /** Launch sizes recorded by saved results (by configuration index), for replays. */
+    void SetLaunchSizes(std::map<uint64_t, double> sizes);
 
 private:
     struct Block
@@ -84,6 +90,8 @@ private:
     Nanoseconds m_BestDuration;
     bool m_SearcherLive;
     mutable std::mt19937_64 m_Random;
+    std::map<uint64_t, double> m_LaunchSizes;
+    bool m_SavedLaunchSizes = false;
 
     void Enumerate(Block& block) const;
     KernelConfiguration Compose(const Block& block, const std::vector<uint32_t>& row) const;

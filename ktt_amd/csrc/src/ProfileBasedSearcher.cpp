@@ -223,6 +223,35 @@ Features CdnaBottlenecks(const Features& f, const ProfileSearcherOptions& option
     return bn;
 }
 
+// CDNA features that count work per launched thread (instructions, quad-cycles, cache requests).
+// A knowledge base profiled at another problem size launched other thread counts; these scale to
+// this search's launch by the ratio of the threads. The bytes moved from memory, the write
+// requests of the output, the utilisations and the duration do not grow with launched threads
+// that fall outside the problem, and stay as profiled.
+const char* const kLaunchCounts[] = {"TCP_TCC_READ_REQ_sum", "TCP_TOTAL_CACHE_ACCESSES_sum", "SQ_INSTS_LDS", "SQ_INSTS_VALU_INT32",
+    "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_BRANCH", "SQ_INSTS_VALU", "F32_INSTS", "F64_INSTS", "VMEM_INSTS",
+    "SCALAR_INSTS", "VALU_ISSUE_SLOTS", "SQ_ACTIVE_INST_VALU2", kCdnaQuadCycles};
+
+Features ScaledToLaunch(const Features& known, const double threads)
+{
+    Features scaled = known;
+    const double recorded = Get(known, "Global size");
+    if (recorded <= 0.0 || threads <= 0.0 || !IsCdna(known))
+    {
+        return scaled;
+    }
+    const double factor = threads / recorded;
+    for (const char* name : kLaunchCounts)
+    {
+        const auto found = scaled.find(name);
+        if (found != scaled.end())
+        {
+            found->second *= factor;
+        }
+    }
+    return scaled;
+}
+
 double Above(const double value, const double threshold)
 {
     return (value - threshold) / (1.0 - threshold);
@@ -662,6 +691,28 @@ void ProfileBasedSearcher::NextBatch(const KernelResult& profiled)
             for (const auto& candidate : candidates)
             {
                 pointers.push_back(KnowledgeOf(candidate));
+            }
+            // Counts of the knowledge base scale to this search's launches when the launch of the
+            // profiled configuration and of every candidate the base holds is known.
+            std::vector<double> threads(candidates.size() + 1, 0.0);
+            bool known = current != nullptr && GetLaunchSize(m_BestOfBatch, threads.back());
+            for (size_t i = 0; known && i < candidates.size(); ++i)
+            {
+                known = pointers[i] == nullptr || GetLaunchSize(candidates[i], threads[i]);
+            }
+            if (known)
+            {
+                estimates.reserve(candidates.size() + 1);
+                for (size_t i = 0; i < candidates.size(); ++i)
+                {
+                    if (pointers[i] != nullptr)
+                    {
+                        estimates.push_back(ScaledToLaunch(*pointers[i], threads[i]));
+                        pointers[i] = &estimates.back();
+                    }
+                }
+                estimates.push_back(ScaledToLaunch(*current, threads.back()));
+                current = &estimates.back();
             }
         }
         else

@@ -67,6 +67,11 @@ uint64_t Searcher::GetUnexploredConfigurationsCount() const
 const std::set<uint64_t>& Searcher::GetExploredIndices() const { return m_Space->Explored(); }
 bool Searcher::IsInitialized() const { return m_Space != nullptr; }
 
+bool Searcher::GetLaunchSize(const KernelConfiguration& configuration, double& threads) const
+{
+    return m_Space != nullptr && m_Space->LaunchSize(configuration, threads);
+}
+
 void Searcher::Initialize(const Exploration& space)
 {
     m_Space = &space;

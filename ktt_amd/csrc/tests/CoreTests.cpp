@@ -176,6 +176,49 @@ void TestSpaceSizes()
     }
 }
 
+/** Launch sizes a searcher sees: the first definition's global x local size after the thread
+ *  modifiers; after a replay, the sizes the saved results recorded. */
+void TestLaunchSizes(const std::vector<KernelResult>& results)
+{
+    ArgumentRegistry arguments;
+    KernelRegistry manager(arguments);
+    Kernel& kernel = manager.GetKernel(AddReferenceCoulomb3dSpace(manager, false));
+    const KernelDefinitionId definition = kernel.GetDefinitions().front()->GetId();
+    kernel.AddThreadModifier(ModifierType::Local, ModifierDimension::X, {"WORK_GROUP_SIZE_X"}, {definition},
+        [](const uint64_t size, const std::vector<uint64_t>& v) { return size * v[0]; });
+    kernel.AddThreadModifier(ModifierType::Global, ModifierDimension::Z, {"Z_ITERATIONS"}, {definition},
+        [](const uint64_t size, const std::vector<uint64_t>& v) { return size / v[0]; });
+    DeterministicSearcher searcher;
+    Exploration exploration(kernel, searcher);
+    for (uint64_t index = 0; index < exploration.Size(); index += 11)
+    {
+        const KernelConfiguration configuration = exploration.ConfigurationAt(index);
+        const auto& pairs = configuration.GetPairs();
+        const double wgx = static_cast<double>(ParameterPair::GetParameterValue<uint64_t>(pairs, "WORK_GROUP_SIZE_X"));
+        const double z = static_cast<double>(ParameterPair::GetParameterValue<uint64_t>(pairs, "Z_ITERATIONS"));
+        double threads = 0.0;
+        CHECK(searcher.GetLaunchSize(configuration, threads));
+        CHECK(threads == 256.0 * 256.0 * (256.0 / z) * wgx);
+    }
+    exploration.SetLaunchSizes({{0, 1234.0}});
+    double threads = 0.0;
+    CHECK(searcher.GetLaunchSize(exploration.ConfigurationAt(0), threads) && threads == 1234.0);
+    CHECK(!searcher.GetLaunchSize(exploration.ConfigurationAt(1), threads));
+
+    // A replay takes the sizes of the saved runs, whatever the kernel's own sizes.
+    KernelRegistry replayManager(arguments);
+    const Kernel& replayKernel = replayManager.GetKernel(AddReferenceCoulomb3dSpace(replayManager, false));
+    DeterministicSearcher replaySearcher;
+    Exploration replay(replayKernel, replaySearcher);
+    ReplayResults(replay, replayKernel, results, nullptr);
+    for (size_t i = 0; i < results.size(); i += 13)
+    {
+        const ComputationResult& first = results[i].GetResults().front();
+        const double recorded = static_cast<double>(first.GetGlobalSize().GetTotalSize()) * static_cast<double>(first.GetLocalSize().GetTotalSize());
+        CHECK(replaySearcher.GetLaunchSize(results[i].GetConfiguration(), threads) && threads == recorded);
+    }
+}
+
 void TestFixtureSpaceAndBest(const std::vector<KernelResult>& results)
 {
     CHECK(results.size() == 210);
@@ -559,6 +602,7 @@ int main(int argc, char** argv)
     TestSpaceSizes();
     const auto results = LoadFixture(argv[1]);
     TestFixtureSpaceAndBest(results);
+    TestLaunchSizes(results);
     TestRandomAndMcmc(results);
     TestStopConditions(results);
     TestJsonRoundTrip(results);

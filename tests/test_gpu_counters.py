@@ -90,3 +90,17 @@ def test_profile_based_search(tmp_path):
     with_counters = [r for r in saved["Results"] if any(c["Name"] == "VALUBusy" for c in
                                                         r["ComputationResults"][0].get("ProfilingData", {}).get("Counters", []))]
     assert len(with_counters) == out["profiled"]
+
+
+def test_profile_based_search_with_a_smaller_grid_knowledge_base(tmp_path):
+    """A knowledge base profiled on a 32^3 grid drives a search on 64^3. The searcher scales the
+    base's instruction counts to each candidate's launch on 64^3, which it computes from the
+    tuning space's thread modifiers."""
+    knowledge = tmp_path / "knowledge"
+    base = run("profile-all", "30", "32", "256", str(knowledge), "", "reduced")
+    assert base["tested"] == 30 and base["profiled"] == 30
+    results = tmp_path / "steered"
+    out = run("search", "15", "64", "512", str(results), str(knowledge) + ".json", "reduced")
+    assert out["tested"] == 15
+    assert out["valid"] == 15
+    assert out["profiled"] >= 2

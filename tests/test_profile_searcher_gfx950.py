@@ -7,9 +7,10 @@ make_coulomb3d_gfx950_counters.py from ProfiledCoulombSum3d profile-all). The ta
 Mean steps to the fastest configuration over 100 seeds, as measured: Random 133.7, MCMC 67.8,
 profile-based learning from its own profiled runs 121.8, with the target's own counters as
 knowledge base 103.0, with 128^3 x 1,024 atoms as knowledge base 105.0, with 64^3 x 1,024 atoms
-224.1. A 64^3 launch has too few workgroups to fill 256 CUs, so its occupancy and busy counters
-rank the shapes for another bottleneck than the target's: a knowledge base must come from a
-launch of the target's grid; fewer atoms are enough."""
+88.7. On the 64^3 grid a shape's workgroups overhang the grid, so its instruction counts grow
+with its launched threads, not with the target's; the searcher scales a knowledge base's counts
+by the ratio of the target's launched threads to the base's (identity on the same grid). Before
+it did, the 64^3 knowledge base took 224.1 steps, more than random search."""
 
 from __future__ import annotations
 
@@ -76,6 +77,19 @@ def test_same_grid_knowledge_base_beats_random(tables):
     print(f"random {random:.1f}, knowledge 128^3 x 1024 {with_same_grid:.1f}, knowledge = target {with_itself:.1f}")
     assert with_same_grid < 0.85 * random
     assert with_itself < 0.85 * random
+
+
+def test_small_grid_knowledge_base_scales_to_the_target_launch(tables):
+    """A knowledge base profiled on a grid 8 times smaller, whose launches overhang it by shape,
+    steers the search once its counts are scaled to the target's launched threads."""
+    target, best_ms, _, small_grid = tables
+    seeds = range(1, 51)
+    random = float(np.mean([steps_to_best(capi.simulate_search(target, "Random", None, seed=seed, compute_units=CU_COUNT), best_ms)
+                            for seed in seeds]))
+    with_small_grid = float(np.mean([steps_to_best(capi.simulate_search(target, "ProfileBased", small_grid, seed=seed,
+                                                                        compute_units=CU_COUNT), best_ms) for seed in seeds]))
+    print(f"random {random:.1f}, knowledge 64^3 x 1024 {with_small_grid:.1f}")
+    assert with_small_grid < 0.85 * random
 
 
 def test_every_searcher_finds_the_best_of_the_space(tables):
