@@ -18,8 +18,9 @@
   *     reference's exact-mapping variant does (scoreTuningConfigurationsExact). Its CDNA counts
   *     that grow with launched threads (instructions, quad-cycles, cache requests) are scaled by
   *     the ratio of a configuration's threads in this search's launch (Searcher::GetLaunchSize)
-  *     to its threads in the base's, so a base profiled on a smaller problem, whose launches
-  *     overhang it by shape, carries over;
+  *     to its threads in the base's, and its device-wide utilisations by the ratio of the compute
+  *     units the two launches occupy (when computeUnits is known), so a base profiled on a
+  *     smaller problem, whose launches overhang it by shape, carries over;
   *   * otherwise the configurations profiled so far in this search: a candidate takes the
   *     counters of the nearest profiled configuration (fewest differing parameter values).
   *

@@ -232,7 +232,11 @@ const char* const kLaunchCounts[] = {"TCP_TCC_READ_REQ_sum", "TCP_TOTAL_CACHE_AC
     "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_BRANCH", "SQ_INSTS_VALU", "F32_INSTS", "F64_INSTS", "VMEM_INSTS",
     "SCALAR_INSTS", "VALU_ISSUE_SLOTS", "SQ_ACTIVE_INST_VALU2", kCdnaQuadCycles};
 
-Features ScaledToLaunch(const Features& known, const double threads)
+// CDNA utilisations averaged over the device: compute units a launch leaves without a workgroup
+// count as idle, so these scale with the share of the units each launch occupies.
+const char* const kDeviceShares[] = {"VALUBusy", "SALUBusy", "OccupancyPercent", "LdsUtil", "SIMD_UTILIZATION"};
+
+Features ScaledToLaunch(const Features& known, const double threads, const double computeUnits)
 {
     Features scaled = known;
     const double recorded = Get(known, "Global size");
@@ -247,6 +251,20 @@ Features ScaledToLaunch(const Features& known, const double threads)
         if (found != scaled.end())
         {
             found->second *= factor;
+        }
+    }
+    const double local = Get(known, "Local size");
+    if (local > 0.0 && computeUnits > 0.0)
+    {
+        const double occupied = std::min(1.0, recorded / local / computeUnits);
+        const double target = std::min(1.0, threads / local / computeUnits);
+        for (const char* name : kDeviceShares)
+        {
+            const auto found = scaled.find(name);
+            if (found != scaled.end())
+            {
+                found->second *= target / occupied;
+            }
         }
     }
     return scaled;
@@ -707,11 +725,11 @@ void ProfileBasedSearcher::NextBatch(const KernelResult& profiled)
                 {
                     if (pointers[i] != nullptr)
                     {
-                        estimates.push_back(ScaledToLaunch(*pointers[i], threads[i]));
+                        estimates.push_back(ScaledToLaunch(*pointers[i], threads[i], m_Options.computeUnits));
                         pointers[i] = &estimates.back();
                     }
                 }
-                estimates.push_back(ScaledToLaunch(*current, threads.back()));
+                estimates.push_back(ScaledToLaunch(*current, threads.back(), m_Options.computeUnits));
                 current = &estimates.back();
             }
         }

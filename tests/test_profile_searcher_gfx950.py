@@ -7,10 +7,12 @@ make_coulomb3d_gfx950_counters.py from ProfiledCoulombSum3d profile-all). The ta
 Mean steps to the fastest configuration over 100 seeds, as measured: Random 133.7, MCMC 67.8,
 profile-based learning from its own profiled runs 121.8, with the target's own counters as
 knowledge base 103.0, with 128^3 x 1,024 atoms as knowledge base 105.0, with 64^3 x 1,024 atoms
-88.7. On the 64^3 grid a shape's workgroups overhang the grid, so its instruction counts grow
-with its launched threads, not with the target's; the searcher scales a knowledge base's counts
-by the ratio of the target's launched threads to the base's (identity on the same grid). Before
-it did, the 64^3 knowledge base took 224.1 steps, more than random search."""
+76.6. On the 64^3 grid a shape's workgroups overhang the grid, so its instruction counts grow
+with its launched threads, not with the target's, and a shape of few workgroups leaves compute
+units idle; the searcher scales a knowledge base's counts by the ratio of the target's launched
+threads to the base's, and its device-wide utilisations by the ratio of the compute units the
+two launches occupy (identity on the same grid). Unscaled, the 64^3 knowledge base took 224.1
+steps, more than random search."""
 
 from __future__ import annotations
 
