@@ -15,6 +15,7 @@
 //        (ISSUE_PHASES 0); "reduced-phases" takes the same shapes with ISSUE_PHASES 0 and 1 (576).
 // Counters are enabled before the tuner initialises the HIP runtime.
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <random>
 #include <sstream>
@@ -92,7 +93,11 @@ int main(int argc, char** argv)
     ktt::Tuner tuner(0, 0, ktt::ComputeApi::HIP);
     tuner.SetGlobalSizeType(ktt::GlobalSizeType::CUDA);
     tuner.SetTimeUnit(ktt::TimeUnit::Microseconds);
-    ktt::Tuner::SetLoggingLevel(ktt::LoggingLevel::Warning);
+    // Quiet by default; KTT_LOG_LEVEL=Info prints each configuration as it runs.
+    if (std::getenv("KTT_LOG_LEVEL") == nullptr)
+    {
+        ktt::Tuner::SetLoggingLevel(ktt::LoggingLevel::Warning);
+    }
     const std::string directory = ktt_hip_kernel_directory();
     const auto definition = tuner.AddKernelDefinitionFromFile("directCoulombSum", directory + "/CoulombSum3d.hip",
         ktt::DimensionVector(gridSize, gridSize * gridSize, 1), ktt::DimensionVector(1, 1, 1));

@@ -176,3 +176,21 @@ def test_dual_issue_counters_steer_the_profile_searcher(phase_tables):
     print(f"random {random:.1f}, profile-based with dual-issue counters {with_pairing:.1f}, without {without_pairing:.1f}")
     assert with_pairing < 0.5 * without_pairing
     assert with_pairing < 0.25 * random
+
+
+def test_small_grid_knowledge_base_on_the_phases_space(tmp_path):
+    """The 576 configurations with both inner loops, steered by a knowledge base profiled on a
+    64^3 grid with 1,024 atoms (coulomb3d_gfx950_64x1024_phases_counters.json). Measured over 100
+    seeds: 135.0 steps to the fastest with the base scaled to the target's launches, 426.1 with
+    the same base given the target's launch sizes (so that nothing is scaled), 276.9 at random."""
+    target, best_ms = write_table("coulomb3d_gfx950_128x8192_phases_counters.json", tmp_path)
+    small_grid, _ = write_table("coulomb3d_gfx950_64x1024_phases_counters.json", tmp_path)
+
+    def mean(searcher, knowledge):
+        return float(np.mean([steps_to_best(capi.simulate_search(target, searcher, knowledge, seed=seed, compute_units=CU_COUNT),
+                                            best_ms) for seed in PHASES_SEEDS]))
+
+    random = mean("Random", None)
+    steered = mean("ProfileBased", small_grid)
+    print(f"random {random:.1f}, knowledge 64^3 x 1024 {steered:.1f}")
+    assert steered < 0.7 * random
