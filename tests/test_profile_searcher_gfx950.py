@@ -143,7 +143,8 @@ def test_phase_tables_hold_the_dual_issue_counters():
     half the other instructions issue in a pair (none when a launch has fewer waves than the
     GPU's 1,024 SIMDs can pair), and the phased loop pairs more than the interleaved loop of the
     same shape."""
-    for name in ("coulomb3d_gfx950_128x8192_phases_counters.json", "coulomb3d_gfx950_128x1024_phases_counters.json"):
+    for name in ("coulomb3d_gfx950_128x8192_phases_counters.json", "coulomb3d_gfx950_128x1024_phases_counters.json",
+                 "coulomb3d_gfx950_64x1024_phases_counters.json"):
         table = json.loads((GOLDEN / name).read_text())
         assert len(table["Rows"]) == 576
         column = {name: i for i, name in enumerate(table["Columns"])}
